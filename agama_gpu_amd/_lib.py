@@ -1,0 +1,93 @@
+"""Loader for the gfx950 C-ABI library (``libagama_gpu.so``).
+
+The library is built in-tree by ``agama_gpu_amd/build.py`` (hipcc,
+--offload-arch=gfx950).  There is no CPU fallback: if the library is missing
+or no GPU is visible, evaluation raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import threading
+from pathlib import Path
+
+import torch  # noqa: F401  -- loads the HIP runtime the library binds to
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libagama_gpu.so"
+
+_lock = threading.Lock()
+_lib = None
+
+
+class AgxError(RuntimeError):
+    """Error code returned by the native library."""
+
+
+class agx_multipole(C.Structure):
+    _fields_ = [
+        ("poly_km", C.c_void_p), ("prog_i", C.c_void_p), ("prog_d", C.c_void_p),
+        ("n_int", C.c_int32), ("n_lm", C.c_int32), ("n_groups", C.c_int32),
+        ("n_steps", C.c_int32), ("lmax", C.c_int32),
+        ("log_scaling", C.c_int32), ("inner_log", C.c_int32), ("outer_log", C.c_int32),
+        ("logr_min", C.c_double), ("dlogr", C.c_double), ("inv_dlogr", C.c_double),
+        ("invPhi0", C.c_double),
+        ("inner_s", C.c_double), ("inner_U", C.c_double), ("inner_W", C.c_double),
+        ("outer_s", C.c_double), ("outer_U", C.c_double), ("outer_W", C.c_double),
+    ]
+
+
+AGX_PHI, AGX_FORCE, AGX_DERIV = 1, 2, 4
+
+# (name, restype, argtypes) of every exported entry point declared in
+# include/agama_gpu.h
+_SIGNATURES = {
+    "agx_version": (C.c_int, []),
+    "agx_error_string": (C.c_char_p, [C.c_int]),
+    "agx_multipole_eval": (C.c_int, [C.POINTER(agx_multipole), C.c_void_p, C.c_int64,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "agx_multipole_density": (C.c_int, [C.POINTER(agx_multipole), C.c_void_p, C.c_int64,
+                                        C.c_double, C.c_void_p, C.c_void_p]),
+}
+
+
+def library_path() -> Path:
+    return LIB_PATH
+
+
+def load():
+    """Load (once, thread-safe) and return the ctypes handle."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not LIB_PATH.exists():
+            raise ImportError(
+                f"{LIB_PATH} is missing: build the gfx950 library first "
+                "(python -c 'import __graft_entry__ as g; g.build()' or "
+                "python agama_gpu_amd/build.py)")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str = "agx call") -> None:
+    if rc != 0:
+        msg = load().agx_error_string(rc).decode()
+        raise AgxError(f"{what} failed: {msg} (code {rc})")
+
+
+def stream_handle(device: torch.device):
+    """hipStream_t of torch's current stream on ``device`` (as void*)."""
+    return C.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def require_gpu() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("agama_gpu_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())

@@ -1,0 +1,38 @@
+// Shared device/host helpers for the gfx950 potential kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "agama_gpu.h"
+
+namespace agx {
+
+// Loads through the constant address space: a wave-uniform address becomes a
+// scalar (SMEM) load into SGPRs, a divergent one a vector load.
+typedef __attribute__((address_space(4))) const double cdouble_t;
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(4))) const dvec2 cdvec2_t;
+typedef __attribute__((address_space(4))) const int cint_t;
+
+__device__ __forceinline__ double cld(const double *p, int i)
+{
+    return ((cdouble_t *)p)[i];
+}
+__device__ __forceinline__ int cldi(const int *p, int i)
+{
+    return ((cint_t *)p)[i];
+}
+
+constexpr int kBlock = 256;
+
+// Grid size for a grid-stride launch over n items: enough blocks to fill
+// 256 CUs several times over, never more than needed.
+inline unsigned grid_for(int64_t n, int per_cu_blocks = 8)
+{
+    int64_t want = (n + kBlock - 1) / kBlock;
+    int64_t cap = (int64_t)256 * per_cu_blocks;
+    if (want > cap) want = cap;
+    if (want < 1) want = 1;
+    return (unsigned)want;
+}
+
+}  // namespace agx

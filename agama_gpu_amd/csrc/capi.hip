@@ -1,0 +1,19 @@
+// Library-level entry points of the C ABI (version, error text).
+#include "agx_common.h"
+
+extern "C" int agx_version(void) { return 10000; }  // 1.0.0
+
+extern "C" const char *agx_error_string(int code)
+{
+    switch (code) {
+    case AGX_OK: return "success";
+    case AGX_E_NULL: return "a required pointer argument is NULL";
+    case AGX_E_SIZE: return "a size argument is out of range";
+    case AGX_E_LMAX: return "lmax exceeds the supported maximum of 32";
+    case AGX_E_LAUNCH: return "kernel launch failed";
+    case AGX_E_WHAT: return "no output requested, or an unknown output flag";
+    case AGX_E_TYPE: return "unknown potential component type";
+    case AGX_E_WORKSPACE: return "workspace too small";
+    default: return "unknown error code";
+    }
+}

@@ -1,0 +1,98 @@
+"""MultipolePotentialGPU -- Agama Multipole BFE on gfx950.
+
+Mirrors the reference class ``MultipolePotentialGPU``
+(gpu_potential.py:633-935): construction from a ``MultipoleCoefs`` table
+(``prune_threshold`` kwarg, ``from_file``), and the Agama API.  Tables are
+built once on the host (multipole_prep.build_multipole_data) and held as
+torch tensors on the device; evaluation is one launch of the gfx950 kernel
+through the C ABI (include/agama_gpu.h: agx_multipole_eval /
+agx_multipole_density) on torch's current stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+from typing import Union
+
+import numpy as np
+import torch
+
+from . import _lib
+from .base import PotentialBase, alloc_outputs, ptr
+from .coefs import MultipoleCoefs, read_coefs
+from .multipole_prep import INV_4PIG, build_multipole_data
+
+
+class MultipolePotentialGPU(PotentialBase):
+    """GPU evaluator for an Agama Multipole expansion (lmax <= 32)."""
+
+    def __init__(self, coefs: MultipoleCoefs, prune_threshold: float = 1e-16, device=None):
+        super().__init__(device)
+        d = build_multipole_data(coefs, prune_threshold=prune_threshold)
+        self._data = d
+        prog = d["program"]
+        dev = self.device
+        self._poly = torch.as_tensor(d["poly_km"].ravel(), device=dev)
+        prog_i = np.concatenate([prog.grp_m, prog.grp_step0, prog.step_cos, prog.step_sin,
+                                 prog.step_l]).astype(np.int32)
+        prog_d = np.concatenate([prog.qmm, prog.step_alpha, prog.step_beta]).astype(np.float64)
+        self._prog_i = torch.as_tensor(prog_i, device=dev)
+        self._prog_d = torch.as_tensor(prog_d, device=dev)
+        s = _lib.agx_multipole()
+        s.poly_km = self._poly.data_ptr()
+        s.prog_i = self._prog_i.data_ptr()
+        s.prog_d = self._prog_d.data_ptr()
+        s.n_int = d["n_intervals"]
+        s.n_lm = d["n_lm"]
+        s.n_groups = len(prog.grp_m)
+        s.n_steps = len(prog.step_l)
+        s.lmax = d["lmax"]
+        s.log_scaling = d["log_scaling"]
+        s.inner_log = d["inner_log"]
+        s.outer_log = d["outer_log"]
+        for k in ("logr_min", "dlogr", "inv_dlogr", "invPhi0", "inner_s", "inner_U", "inner_W",
+                  "outer_s", "outer_U", "outer_W"):
+            setattr(s, k, d[k])
+        self._desc = s
+        self._inv_4piG = INV_4PIG
+        self.lmax = d["lmax"]
+        self.n_lm = d["n_lm"]
+        self.n_intervals = d["n_intervals"]
+
+    # ---- constructors -------------------------------------------------------
+    @classmethod
+    def from_file(cls, path: Union[str, Path], **kw) -> "MultipolePotentialGPU":
+        """Load from an Agama Multipole coefficient file (needs #dPhi/dr)."""
+        c = read_coefs(Path(path))
+        if not isinstance(c, MultipoleCoefs):
+            raise ValueError(f"{path} is not a Multipole coefficient file")
+        return cls(c, **kw)
+
+    def __repr__(self):
+        return f"MultipolePotentialGPU(lmax={self.lmax}, n_lm={self.n_lm}, n_int={self.n_intervals})"
+
+    @property
+    def descriptor(self):
+        """The C-ABI descriptor (agx_multipole) of this potential's tables."""
+        return self._desc
+
+    # ---- primitives ---------------------------------------------------------
+    def _evaluate(self, xyz, want_phi, want_force, want_deriv, t=0.0, out=None):
+        n = xyz.shape[0]
+        phi, force, deriv = alloc_outputs(n, self.device, want_phi, want_force, want_deriv, out)
+        what = (_lib.AGX_PHI if want_phi else 0) | (_lib.AGX_FORCE if want_force else 0) | \
+               (_lib.AGX_DERIV if want_deriv else 0)
+        L = _lib.load()
+        rc = L.agx_multipole_eval(C.byref(self._desc), xyz.data_ptr(), n, ptr(phi), ptr(force),
+                                  ptr(deriv), what, _lib.stream_handle(self.device))
+        _lib.check(rc, "agx_multipole_eval")
+        return phi, force, deriv
+
+    def _density(self, xyz, t=0.0):
+        n = xyz.shape[0]
+        rho = torch.empty(n, dtype=torch.float64, device=self.device)
+        rc = _lib.load().agx_multipole_density(C.byref(self._desc), xyz.data_ptr(), n,
+                                               self._inv_4piG, rho.data_ptr(),
+                                               _lib.stream_handle(self.device))
+        _lib.check(rc, "agx_multipole_density")
+        return rho

@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Benchmark: FP64 Multipole potential+force throughput on one MI355X.
+
+Workload: Multipole halo, lmax=8, 50 log radial nodes on [0.1, 500] kpc, from
+a triaxial Dehnen density (gamma=1, M=1e12, a=20 kpc, axes 1:0.8:0.6; 15
+active harmonics after symmetry pruning), potential+force at N=10M positions
+(Hernquist a=20 kpc, seed 0, 1% log-uniform in [0.01, 2000] kpc).  Inputs and
+tables are resident in HBM before the timed region; one step = one launch
+over all N particles.
+
+Prints one JSON line (rank 0).  Extra keys: ``roofline`` (FP64 VALU bound,
+algorithmic flops per launch / mean kernel time) and ``cpu_baseline`` (the
+oracle's OpenMP C restatement on this host's cores, same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 vector peak
+PEAK_HBM_GBS = 8000.0
+
+
+def opcount_multipole(desc: dict, kinds: np.ndarray, order: int = 1) -> float:
+    """Algorithmic FP64 flops (FMA = 2, each transcendental/sqrt/divide = 1)
+    for one launch, summed over particles by the branch each one takes.
+
+    Interior point, pot+force: 80 + 40 n + 12 S + 5 G, with n active (l,m)
+    terms including (0,0), G active |m| groups and S = sum over groups of
+    (l_last - |m|) recurrence steps; pot only: 60 + 16 n + 6 S; evalDeriv:
+    350 + 90 n + 12 S + 5 G.  Extrapolated point: 40 (pot+force), 60
+    (evalDeriv), 20 (pot).
+    """
+    prog = desc["program"]
+    n = desc["n_lm"]
+    G = len(prog.grp_m)
+    S = int(sum(int(prog.step_l[prog.grp_step0[g + 1] - 1]) - int(prog.grp_m[g]) for g in range(G)))
+    if order == 0:
+        f_in, f_ex = 60 + 16 * n + 6 * S, 20
+    elif order == 1:
+        f_in, f_ex = 80 + 40 * n + 12 * S + 5 * G, 40
+    else:
+        f_in, f_ex = 350 + 90 * n + 12 * S + 5 * G, 60
+    n_in = int(np.count_nonzero(kinds == 2))
+    return float(f_in * n_in + f_ex * (len(kinds) - n_in)), dict(n=n, G=G, S=S, f_interior=f_in,
+                                                                   f_extrap=f_ex, n_interior=n_in)
+
+
+def classify(desc: dict, x: np.ndarray) -> np.ndarray:
+    r = np.sqrt((x * x).sum(1))
+    kinds = np.full(len(x), 2, np.int8)
+    with np.errstate(divide="ignore"):
+        lr = np.log(r)
+    kinds[lr < desc["logr_min"]] = 1
+    k = np.floor((lr - desc["logr_min"]) * desc["inv_dlogr"])
+    kinds[(kinds == 2) & (k >= desc["n_intervals"])] = 3
+    kinds[r < 1e-300] = 0
+    return kinds
+
+
+def cpu_threads() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model() -> str:
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=10_000_000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): evaluation of "
+              "independent particles shards by contiguous ranges with replicated tables; "
+              "no multi-GPU figure is produced here.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from agama_gpu_amd import MultipolePotentialGPU
+    from agama_gpu_amd.workloads import halo_lmax8_coefs, hernquist_points
+
+    N = args.n
+    coefs = halo_lmax8_coefs()
+    x_np = hernquist_points(N, 0)
+    dev = torch.device("cuda", 0)
+    pot = MultipolePotentialGPU(coefs, device=dev)
+    desc = pot._data
+    kinds = classify(desc, x_np)
+    flops, opinfo = opcount_multipole(desc, kinds, order=1)
+
+    xyz = torch.as_tensor(x_np, device=dev)
+    phi = torch.empty(N, dtype=torch.float64, device=dev)
+    force = torch.empty((N, 3), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    def step():
+        pot.eval_into(xyz, phi=phi, force=force)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream(dev)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for a, b in evs:
+        a.record(stream)
+        step()
+        b.record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    kern_ms = [a.elapsed_time(b) for a, b in evs]
+    k_mean = float(np.mean(kern_ms))
+    value = N / (ms_step * 1e-3)
+    achieved = flops / (k_mean * 1e-3) / 1e12
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle.oracle import OracleMultipole
+        orc = OracleMultipole.from_coefs(coefs)
+        th = cpu_threads()
+        orc.pot_force(x_np[:100_000], th)
+        ts = []
+        for _ in range(args.cpu_reps):
+            c0 = time.perf_counter()
+            orc.pot_force(x_np, th)
+            ts.append(time.perf_counter() - c0)
+        t_cpu = float(np.median(ts))
+        cpu = {"value": N / t_cpu, "unit": "particles/s", "cores": th, "kind": "port",
+               "sample": f"full N={N} workload, median of {args.cpu_reps} after 1 warmup; "
+                         f"oracle/mp_oracle.c -O3 OpenMP on {cpu_model()}"}
+
+    line = {
+        "metric": "FP64 particles/s (Multipole lmax=8, pot+force, N=10M); batched+grad vs plain",
+        "value": value,
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "multipole_halo_lmax8_pot_force", "n_particles": N,
+                   "lmax": 8, "radial_nodes": 50, "active_terms": opinfo["n"],
+                   "table": "triaxial Dehnen gamma=1 M=1e12 a=20 axes 1:0.8:0.6",
+                   "points": "Hernquist a=20 seed 0, 1% log-uniform [0.01,2000]"},
+        "roofline": {"bound": "valu_fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel_ms": k_mean, "flops_per_launch": flops, "opcount": opinfo},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,92 @@
+/*
+ * agama_gpu.h -- C ABI of the MI355X (gfx950) Agama-compatible potential
+ * evaluator.  Plain pointers and sizes only: every pointer argument named
+ * `d_*` or documented as "device" is a device (HBM) address, e.g. a torch
+ * tensor's data_ptr(); `stream` is a hipStream_t (NULL = default stream).
+ * Calls are asynchronous on `stream`; the library never allocates on the
+ * evaluation path.
+ *
+ * Output convention (Agama's, reference gpu_potential.py:8-16):
+ *   phi   [N]      potential
+ *   force [N][3]   -grad Phi
+ *   deriv [N][6]   force derivatives [dFx/dx, dFy/dy, dFz/dz, dFx/dy, dFy/dz, dFz/dx]
+ * Input positions are AoS [N][3] FP64 (the reference splits them into SoA
+ * copies first, gpu_potential.py:812-814; reading AoS directly saves those
+ * three copy kernels).
+ *
+ * Return value: 0 on success, a negative AGX_E* code otherwise
+ * (agx_error_string() describes it).
+ */
+#ifndef AGAMA_GPU_H
+#define AGAMA_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    AGX_OK = 0,
+    AGX_E_NULL = -1,        /* required pointer is NULL                        */
+    AGX_E_SIZE = -2,        /* N < 0 or a table dimension out of range         */
+    AGX_E_LMAX = -3,        /* lmax > 32                                       */
+    AGX_E_LAUNCH = -4,      /* kernel launch failed (hipGetLastError)          */
+    AGX_E_WHAT = -5,        /* no output requested / unknown flag              */
+    AGX_E_TYPE = -6,        /* unknown component type in a composite           */
+    AGX_E_WORKSPACE = -7    /* workspace too small                             */
+};
+
+/* output selection flags */
+#define AGX_PHI   1u
+#define AGX_FORCE 2u
+#define AGX_DERIV 4u
+
+/* ------------------------------------------------------------------------
+ * Multipole (spherical-harmonic BFE with quintic log-r splines).
+ *
+ * Tables are built on the host (agama_gpu_amd/multipole_prep.py, restating
+ * the reference's _build_multipole_data, gpu_potential.py:408-626) and
+ * uploaded once; this struct only carries their device addresses and the
+ * scalar parameters.
+ *   poly_km : [n_int][n_lm][6] Horner coefficients a0..a5 of the scaled
+ *             radial functions on interval k, s in [0,1]
+ *   prog_i  : grp_m[n_groups], grp_step0[n_groups+1], step_cos[n_steps],
+ *             step_sin[n_steps], step_l[n_steps]        (int32, device)
+ *   prog_d  : qmm[n_groups], alpha[n_steps], beta[n_steps] (FP64, device)
+ * ---------------------------------------------------------------------- */
+typedef struct agx_multipole {
+    const double *poly_km;
+    const int32_t *prog_i;
+    const double *prog_d;
+    int32_t n_int, n_lm, n_groups, n_steps, lmax;
+    int32_t log_scaling, inner_log, outer_log;
+    double logr_min, dlogr, inv_dlogr, invPhi0;
+    double inner_s, inner_U, inner_W;
+    double outer_s, outer_U, outer_W;
+} agx_multipole;
+
+/* Potential / force / force derivatives at N points.
+ * Replaces multipole_potential_kernel, multipole_force_kernel and
+ * multipole_hess_kernel (reference _multipole_potential_kernel.cu:415-455,
+ * 466-800) as launched by MultipolePotentialGPU._launch_eval/_launch_hess
+ * (gpu_potential.py:757-782).  `what` is any OR of AGX_PHI|AGX_FORCE|AGX_DERIV;
+ * unrequested outputs may be NULL. */
+int agx_multipole_eval(const agx_multipole *mp, const double *d_xyz, int64_t n,
+                       double *d_phi, double *d_force, double *d_deriv,
+                       uint32_t what, void *stream);
+
+/* Density rho = Laplacian(Phi) / (4 pi G) at N points.
+ * Replaces multipole_density_kernel (_multipole_potential_kernel.cu:809-958)
+ * as launched by _launch_density (gpu_potential.py:784-793). */
+int agx_multipole_density(const agx_multipole *mp, const double *d_xyz, int64_t n,
+                          double inv_4piG, double *d_rho, void *stream);
+
+/* Library identification and error text. */
+int agx_version(void);
+const char *agx_error_string(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AGAMA_GPU_H */

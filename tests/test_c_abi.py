@@ -1,0 +1,57 @@
+"""The C-ABI library builds, loads and exports every entry point declared in
+include/agama_gpu.h (no compute calls: runs without a GPU)."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "agama_gpu.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(agx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    assert "agx_multipole_eval" in syms and "agx_version" in syms
+
+
+def test_library_exports_all_declared_symbols():
+    from agama_gpu_amd import _lib
+    path = _lib.library_path()
+    if not path.exists():
+        pytest.fail(f"{path} not built; run __graft_entry__.build()")
+    L = C.CDLL(str(path))
+    missing = [s for s in declared_symbols() if not hasattr(L, s)]
+    assert not missing, missing
+    for s in declared_symbols():
+        assert s in _lib._SIGNATURES, f"{s} has no ctypes signature in _lib"
+
+
+def test_version_and_error_strings():
+    from agama_gpu_amd import _lib
+    L = _lib.load()
+    assert L.agx_version() >= 10000
+    assert b"NULL" in L.agx_error_string(-1)
+    assert L.agx_error_string(12345) == b"unknown error code"
+
+
+def test_argument_validation_without_gpu():
+    """Invalid arguments are rejected on the host before any launch."""
+    from agama_gpu_amd import _lib
+    L = _lib.load()
+    d = _lib.agx_multipole()
+    assert L.agx_multipole_eval(None, None, 0, None, None, None, 1, None) == -1
+    assert L.agx_multipole_eval(C.byref(d), None, 10, None, None, None, 1, None) == -1
+    d.poly_km = d.prog_i = d.prog_d = 8
+    d.n_int, d.n_lm, d.lmax = 4, 1, 40
+    assert L.agx_multipole_eval(C.byref(d), 8, 10, 8, None, None, 1, None) == -3
+    d.lmax = 0
+    assert L.agx_multipole_eval(C.byref(d), 8, -1, 8, None, None, 1, None) == -2
+    assert L.agx_multipole_eval(C.byref(d), 8, 10, 8, None, None, 0, None) == -5
+    assert L.agx_multipole_eval(C.byref(d), 8, 10, None, None, None, 1, None) == -1
+    assert L.agx_multipole_eval(C.byref(d), 8, 0, 8, None, None, 1, None) == 0
