@@ -317,8 +317,12 @@ __device__ __forceinline__ void extrap(const MPK &P, const Where &w, double x, d
                                        PointOut &o, double inv4piG)
 {
     double Pv, D, D2;
-    if (w.kind == 0) {
+    if (w.kind == 0) {      // origin: Phi = W, zero force and Hessian (reference :166-171)
         o.phi = (ORD == ORD_DENS) ? 0.0 : P.inner_W;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) o.g[a] = 0.0;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) o.h[a] = 0.0;
         return;
     }
     if (w.kind == 1) {
@@ -411,10 +415,10 @@ extern "C" int agx_multipole_eval(const agx_multipole *mp, const double *d_xyz, 
     int rc = check_mp(mp, d_xyz, n);
     if (rc) return rc;
     if (what == 0 || (what & ~(AGX_PHI | AGX_FORCE | AGX_DERIV))) return AGX_E_WHAT;
+    if (n == 0) return AGX_OK;   // empty input: nothing to write, outputs may be NULL
     if (((what & AGX_PHI) && !d_phi) || ((what & AGX_FORCE) && !d_force) ||
         ((what & AGX_DERIV) && !d_deriv))
         return AGX_E_NULL;
-    if (n == 0) return AGX_OK;
     MPK P = make_args(mp);
     hipStream_t s = (hipStream_t)stream;
     double *phi = (what & AGX_PHI) ? d_phi : nullptr;
@@ -435,8 +439,8 @@ extern "C" int agx_multipole_density(const agx_multipole *mp, const double *d_xy
 {
     int rc = check_mp(mp, d_xyz, n);
     if (rc) return rc;
-    if (!d_rho) return AGX_E_NULL;
     if (n == 0) return AGX_OK;
+    if (!d_rho) return AGX_E_NULL;
     MPK P = make_args(mp);
     dim3 grid(grid_for(n)), block(kBlock);
     hipLaunchKernelGGL(mp_lane_kernel<ORD_DENS>, grid, block, 0, (hipStream_t)stream, P, d_xyz, n,
