@@ -37,11 +37,18 @@ class agx_multipole(C.Structure):
 
 
 AGX_PHI, AGX_FORCE, AGX_DERIV = 1, 2, 4
+AGX_OPT_MP_PATH, AGX_OPT_MP_TILE, AGX_OPT_MP_BIN_MIN_N, AGX_OPT_MP_BIN_MIN_TERMS = 1, 2, 3, 4
+
+
+def set_option(key: int, value: int) -> None:
+    """Process-wide tuning knob of the native library (see agx_set_option)."""
+    check(load().agx_set_option(int(key), int(value)), "agx_set_option")
 
 # (name, restype, argtypes) of every exported entry point declared in
 # include/agama_gpu.h
 _SIGNATURES = {
     "agx_version": (C.c_int, []),
+    "agx_set_option": (C.c_int, [C.c_int, C.c_int64]),
     "agx_error_string": (C.c_char_p, [C.c_int]),
     "agx_multipole_eval": (C.c_int, [C.POINTER(agx_multipole), C.c_void_p, C.c_int64,
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),

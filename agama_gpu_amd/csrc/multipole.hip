@@ -360,7 +360,7 @@ __global__ __launch_bounds__(kBlock) void mp_lane_kernel(MPK P, const double *__
     const int64_t stride = (int64_t)gridDim.x * kBlock;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
         const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
-        PointOut o;
+        PointOut o{};
         Where w = classify(P, x, y, z);
         if (w.kind == 2)
             interior<ORD, false>(P, P.poly + (int64_t)w.k * P.n_lm * 6, x, y, z, w.r2, w.r, w.lnr,
@@ -368,6 +368,95 @@ __global__ __launch_bounds__(kBlock) void mp_lane_kernel(MPK P, const double *__
         else
             extrap<ORD>(P, w, x, y, z, o, inv4piG);
         store<ORD>(o, i, phi, force, deriv);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Radially binned evaluation.  A workgroup takes a tile of particles, sorts
+// the tile's indices by radial interval in LDS (counting sort, each interval
+// padded to whole wavefronts), and then every wavefront evaluates 64 slots
+// that share one interval k.  With k wave-uniform, the interval's coefficient
+// slab is read with scalar loads into SGPRs and feeds the FP64 FMAs directly:
+// no per-lane gathers of the table at all.  Bin 0 collects the particles
+// outside the grid (origin, inner and outer laws), which need no table.
+// ---------------------------------------------------------------------------
+constexpr int kBinThreads = 256;
+constexpr int kMaxBins = 128;          // n_int + 1 bins at most
+constexpr int kMaxTile = 8192;         // particles per tile
+constexpr int kMaxSlots = kMaxTile + kMaxBins * 63;
+constexpr int kMaxChunks = kMaxSlots / 64;
+
+template <int ORD>
+__global__ __launch_bounds__(kBinThreads) void mp_binned_kernel(
+    MPK P, const double *__restrict__ xyz, int64_t n, int64_t tile, double *__restrict__ phi,
+    double *__restrict__ force, double *__restrict__ deriv, double inv4piG)
+{
+    __shared__ uint16_t slot_idx[kMaxSlots];
+    __shared__ uint8_t pbin[kMaxTile];
+    __shared__ int cnt[kMaxBins], off[kMaxBins + 1], cur[kMaxBins];
+    __shared__ uint8_t chunk_bin[kMaxChunks];
+    __shared__ int n_chunks_s;
+
+    const int nb = P.n_int + 1;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6, n_waves = kBinThreads / 64;
+    const int64_t n_tiles = (n + tile - 1) / tile;
+
+    for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const int64_t base = t * tile;
+        const int cnt_here = (int)((n - base < tile) ? (n - base) : tile);
+        for (int b = tid; b < nb; b += kBinThreads) { cnt[b] = 0; cur[b] = 0; }
+        __syncthreads();
+        // phase 1: classify and histogram
+        for (int j = tid; j < cnt_here; j += kBinThreads) {
+            const int64_t i = base + j;
+            Where w = classify(P, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+            const int b = (w.kind == 2) ? w.k + 1 : 0;
+            pbin[j] = (uint8_t)b;
+            atomicAdd(&cnt[b], 1);
+        }
+        __syncthreads();
+        // phase 2: wave-padded offsets, chunk -> bin map
+        if (tid == 0) {
+            int o = 0;
+            for (int b = 0; b < nb; ++b) { off[b] = o; o += (cnt[b] + 63) & ~63; }
+            off[nb] = o;
+            n_chunks_s = o >> 6;
+        }
+        __syncthreads();
+        for (int b = tid; b < nb; b += kBinThreads)
+            for (int c = off[b] >> 6; c < (off[b + 1] >> 6); ++c) chunk_bin[c] = (uint8_t)b;
+        for (int s = tid; s < off[nb]; s += kBinThreads) slot_idx[s] = 0xFFFF;
+        __syncthreads();
+        // phase 3: scatter tile-local indices into their bin's slots
+        for (int j = tid; j < cnt_here; j += kBinThreads) {
+            const int b = pbin[j];
+            const int pos = off[b] + atomicAdd(&cur[b], 1);
+            slot_idx[pos] = (uint16_t)j;
+        }
+        __syncthreads();
+        // phase 4: one interval per wavefront chunk
+        const int nch = n_chunks_s;
+        for (int c = wave; c < nch; c += n_waves) {
+            const int b = __builtin_amdgcn_readfirstlane((int)chunk_bin[c]);
+            const int j = slot_idx[c * 64 + lane];
+            if (j == 0xFFFF) continue;
+            const int64_t i = base + j;
+            const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+            PointOut o{};
+            if (b > 0) {
+                const int k = b - 1;
+                const double r2 = fma(x, x, fma(y, y, z * z));
+                const double r = sqrt(r2);
+                interior<ORD, true>(P, P.poly + (int64_t)k * P.n_lm * 6, x, y, z, r2, r, log(r), k,
+                                    o, inv4piG);
+            } else {
+                Where w = classify(P, x, y, z);
+                extrap<ORD>(P, w, x, y, z, o, inv4piG);
+            }
+            store<ORD>(o, i, phi, force, deriv);
+        }
+        __syncthreads();
     }
 }
 
@@ -404,9 +493,83 @@ static int check_mp(const agx_multipole *mp, const double *xyz, int64_t n)
     return AGX_OK;
 }
 
+// Tunables (agx_set_option): evaluation path and binned tile size.
+static int g_mp_path = 0;            // 0 auto, 1 lane kernel, 2 binned kernel
+static int64_t g_mp_tile = 0;        // 0 = automatic
+static int64_t g_mp_bin_min_n = 200000;
+static int64_t g_mp_bin_min_terms = 24;
+
+struct BinLaunch {
+    unsigned grid;
+    int64_t tile;
+};
+
+static BinLaunch bin_launch(int64_t n)
+{
+    const int64_t max_wg = 256 * 3;   // three 42 KB-LDS workgroups per CU
+    int64_t tile = g_mp_tile > 0 ? g_mp_tile : (n + max_wg - 1) / max_wg;
+    if (tile < 1024) tile = 1024;
+    if (tile > kMaxTile) tile = kMaxTile;
+    int64_t n_tiles = (n + tile - 1) / tile;
+    int64_t grid = n_tiles < max_wg ? n_tiles : max_wg;
+    // equal tile counts per workgroup: rebalance the tile size
+    int64_t per = (n_tiles + grid - 1) / grid;
+    tile = (n + grid * per - 1) / (grid * per);
+    if (tile < 1) tile = 1;
+    return BinLaunch{(unsigned)grid, tile};
+}
+
+static bool use_binned(const agx_multipole *mp, int64_t n)
+{
+    if (mp->n_int + 1 > kMaxBins) return false;
+    if (g_mp_path == 1) return false;
+    if (g_mp_path == 2) return true;
+    // Binning pays when the per-term gathers dominate (many active terms);
+    // with few terms its sort costs more than the gathers it removes.
+    return n >= g_mp_bin_min_n && mp->n_lm >= g_mp_bin_min_terms;
+}
+
+template <int ORD>
+static void launch_mp(const agx_multipole *mp, const MPK &P, const double *xyz, int64_t n,
+                      double *phi, double *force, double *deriv, double inv4piG, hipStream_t s)
+{
+    if (use_binned(mp, n)) {
+        BinLaunch bl = bin_launch(n);
+        hipLaunchKernelGGL(mp_binned_kernel<ORD>, dim3(bl.grid), dim3(kBinThreads), 0, s, P, xyz,
+                           n, bl.tile, phi, force, deriv, inv4piG);
+    } else {
+        hipLaunchKernelGGL(mp_lane_kernel<ORD>, dim3(grid_for(n)), dim3(kBlock), 0, s, P, xyz, n,
+                           phi, force, deriv, inv4piG);
+    }
+}
+
 }  // namespace agx
 
 using namespace agx;
+
+extern "C" int agx_set_option(int key, int64_t value)
+{
+    switch (key) {
+    case AGX_OPT_MP_PATH:
+        if (value < 0 || value > 2) return AGX_E_SIZE;
+        g_mp_path = (int)value;
+        return AGX_OK;
+    case AGX_OPT_MP_TILE:
+        if (value < 0 || value > kMaxTile) return AGX_E_SIZE;
+        g_mp_tile = value;
+        return AGX_OK;
+    case AGX_OPT_MP_BIN_MIN_N:
+        if (value < 0) return AGX_E_SIZE;
+        g_mp_bin_min_n = value;
+        return AGX_OK;
+    case AGX_OPT_MP_BIN_MIN_TERMS:
+        if (value < 0) return AGX_E_SIZE;
+        g_mp_bin_min_terms = value;
+        return AGX_OK;
+    default:
+        return AGX_E_WHAT;
+    }
+}
 
 extern "C" int agx_multipole_eval(const agx_multipole *mp, const double *d_xyz, int64_t n,
                                   double *d_phi, double *d_force, double *d_deriv, uint32_t what,
@@ -424,13 +587,12 @@ extern "C" int agx_multipole_eval(const agx_multipole *mp, const double *d_xyz, 
     double *phi = (what & AGX_PHI) ? d_phi : nullptr;
     double *force = (what & AGX_FORCE) ? d_force : nullptr;
     double *deriv = (what & AGX_DERIV) ? d_deriv : nullptr;
-    dim3 grid(grid_for(n)), block(kBlock);
     if (deriv)
-        hipLaunchKernelGGL(mp_lane_kernel<ORD_HESS>, grid, block, 0, s, P, d_xyz, n, phi, force, deriv, 0.0);
+        launch_mp<ORD_HESS>(mp, P, d_xyz, n, phi, force, deriv, 0.0, s);
     else if (force)
-        hipLaunchKernelGGL(mp_lane_kernel<ORD_GRAD>, grid, block, 0, s, P, d_xyz, n, phi, force, nullptr, 0.0);
+        launch_mp<ORD_GRAD>(mp, P, d_xyz, n, phi, force, nullptr, 0.0, s);
     else
-        hipLaunchKernelGGL(mp_lane_kernel<ORD_POT>, grid, block, 0, s, P, d_xyz, n, phi, nullptr, nullptr, 0.0);
+        launch_mp<ORD_POT>(mp, P, d_xyz, n, phi, nullptr, nullptr, 0.0, s);
     return hipGetLastError() == hipSuccess ? AGX_OK : AGX_E_LAUNCH;
 }
 
@@ -442,8 +604,6 @@ extern "C" int agx_multipole_density(const agx_multipole *mp, const double *d_xy
     if (n == 0) return AGX_OK;
     if (!d_rho) return AGX_E_NULL;
     MPK P = make_args(mp);
-    dim3 grid(grid_for(n)), block(kBlock);
-    hipLaunchKernelGGL(mp_lane_kernel<ORD_DENS>, grid, block, 0, (hipStream_t)stream, P, d_xyz, n,
-                       d_rho, nullptr, nullptr, inv_4piG);
+    launch_mp<ORD_DENS>(mp, P, d_xyz, n, d_rho, nullptr, nullptr, inv_4piG, (hipStream_t)stream);
     return hipGetLastError() == hipSuccess ? AGX_OK : AGX_E_LAUNCH;
 }

@@ -82,6 +82,23 @@ int agx_multipole_eval(const agx_multipole *mp, const double *d_xyz, int64_t n,
 int agx_multipole_density(const agx_multipole *mp, const double *d_xyz, int64_t n,
                           double inv_4piG, double *d_rho, void *stream);
 
+/* Tuning options (process-wide; set before launches, not thread-safe):
+ *   AGX_OPT_MP_PATH      Multipole evaluation path: 0 automatic, 1 one particle
+ *                        per lane with per-lane table reads, 2 radially binned
+ *                        (wave-uniform interval, scalar table reads; needs
+ *                        n_int <= 127)
+ *   AGX_OPT_MP_TILE      particles per binned tile (0 = automatic, <= 8192)
+ *   AGX_OPT_MP_BIN_MIN_N smallest N for which the automatic path bins
+ *   AGX_OPT_MP_BIN_MIN_TERMS smallest number of active harmonics for which
+ *                        the automatic path bins
+ * No reference counterpart: the reference has a single launch shape
+ * (gpu_potential.py:160, 764). */
+#define AGX_OPT_MP_PATH 1
+#define AGX_OPT_MP_TILE 2
+#define AGX_OPT_MP_BIN_MIN_N 3
+#define AGX_OPT_MP_BIN_MIN_TERMS 4
+int agx_set_option(int key, int64_t value);
+
 /* Library identification and error text. */
 int agx_version(void);
 const char *agx_error_string(int code);

@@ -46,12 +46,21 @@ def points(n, seed=0):
     return x
 
 
+@pytest.fixture(params=[1, 2], ids=["lane", "binned"])
+def mp_path(request):
+    """Force one evaluation path of the native library for the test."""
+    from agama_gpu_amd import _lib
+    _lib.set_option(_lib.AGX_OPT_MP_PATH, request.param)
+    yield request.param
+    _lib.set_option(_lib.AGX_OPT_MP_PATH, 0)
+
+
 @pytest.mark.parametrize("name", ["fire_dense", "lmc", "halo_lmax8"])
-def test_pot_force_deriv_density(cases, name):
+def test_pot_force_deriv_density(cases, name, mp_path):
     from agama_gpu_amd import MultipolePotentialGPU
     from oracle.oracle import OracleMultipole
     c = cases[name]
-    print(f"case={name} n=100000", flush=True)
+    print(f"case={name} path={mp_path} n=100000", flush=True)
     x = points(100_000, 0)
     gpu = MultipolePotentialGPU(c)
     orc = OracleMultipole.from_coefs(c)
@@ -92,7 +101,7 @@ def test_single_point_and_numpy_input(cases):
     assert np.max(np.abs(f.cpu().numpy() - Fo[0])) < 1e-11 * np.abs(Fo).max()
 
 
-def test_empty_and_ragged_sizes(cases):
+def test_empty_and_ragged_sizes(cases, mp_path):
     from agama_gpu_amd import MultipolePotentialGPU
     from oracle.oracle import OracleMultipole
     gpu = MultipolePotentialGPU(cases["lmc"])
