@@ -10,11 +10,20 @@ from .coefs import CylSplineCoefs, MultipoleCoefs, read_coefs, write_coefs
 from .multipole_prep import G_AGAMA, INV_4PIG
 from .base import PotentialBase
 from .multipole import MultipolePotentialGPU
-from .composite import CompositePotentialGPU
+from .composite import (CompositePotentialGPU, EvolvingPotentialGPU, ScaledPotentialGPU,
+                        ShiftedPotentialGPU)
+from .analytic import (AnalyticPotentialGPU, DehnenSphericalPotentialGPU, DiskAnsatzPotentialGPU,
+                       HernquistPotentialGPU, IsochronePotentialGPU, LogHaloPotentialGPU,
+                       MiyamotoNagaiPotentialGPU, NFWPotentialGPU, PlummerPotentialGPU,
+                       UniformAccelerationGPU)
+from .factory import PotentialGPU
 
 __all__ = [
-    "PotentialBase", "MultipolePotentialGPU", "CompositePotentialGPU",
-    "MultipoleCoefs", "CylSplineCoefs", "read_coefs", "write_coefs",
-    "G_AGAMA", "INV_4PIG",
+    "PotentialGPU", "PotentialBase", "MultipolePotentialGPU", "CompositePotentialGPU",
+    "EvolvingPotentialGPU", "ShiftedPotentialGPU", "ScaledPotentialGPU",
+    "AnalyticPotentialGPU", "NFWPotentialGPU", "PlummerPotentialGPU", "HernquistPotentialGPU",
+    "DehnenSphericalPotentialGPU", "IsochronePotentialGPU", "MiyamotoNagaiPotentialGPU",
+    "LogHaloPotentialGPU", "DiskAnsatzPotentialGPU", "UniformAccelerationGPU",
+    "MultipoleCoefs", "CylSplineCoefs", "read_coefs", "write_coefs", "G_AGAMA", "INV_4PIG",
 ]
 __version__ = "0.1.0"

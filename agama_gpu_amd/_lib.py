@@ -36,6 +36,19 @@ class agx_multipole(C.Structure):
     ]
 
 
+class agx_component(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32), ("pad", C.c_int32), ("p", C.c_double * 8),
+        ("table", C.c_void_p), ("ampl", C.c_double), ("scale", C.c_double),
+        ("center", C.c_double * 3),
+    ]
+
+
+AGX_MAX_COMPONENTS = 16
+(AGX_T_NFW, AGX_T_PLUMMER, AGX_T_HERNQUIST, AGX_T_DEHNEN, AGX_T_ISOCHRONE,
+ AGX_T_MIYAMOTO_NAGAI, AGX_T_LOGHALO, AGX_T_DISK_ANSATZ, AGX_T_UNIFORM_ACCEL) = range(1, 10)
+AGX_T_MULTIPOLE, AGX_T_CYLSPLINE = 16, 17
+
 AGX_PHI, AGX_FORCE, AGX_DERIV = 1, 2, 4
 AGX_OPT_MP_PATH, AGX_OPT_MP_TILE, AGX_OPT_MP_BIN_MIN_N, AGX_OPT_MP_BIN_MIN_TERMS = 1, 2, 3, 4
 
@@ -54,6 +67,13 @@ _SIGNATURES = {
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "agx_multipole_density": (C.c_int, [C.POINTER(agx_multipole), C.c_void_p, C.c_int64,
                                         C.c_double, C.c_void_p, C.c_void_p]),
+    "agx_composite_eval": (C.c_int, [C.POINTER(agx_component), C.c_int, C.c_void_p, C.c_int64,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                     C.c_void_p]),
+    "agx_composite_density": (C.c_int, [C.POINTER(agx_component), C.c_int, C.c_void_p,
+                                        C.c_int64, C.c_void_p, C.c_void_p]),
+    "agx_multipole_pack_size": (C.c_int64, []),
+    "agx_multipole_pack": (C.c_int, [C.POINTER(agx_multipole), C.c_void_p, C.c_int64]),
 }
 
 

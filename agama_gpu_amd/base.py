@@ -54,15 +54,27 @@ class PotentialBase:
     def __init__(self, device=None):
         self.device = torch.device(device) if device is not None else require_gpu()
 
-    # ---- primitives implemented by subclasses ------------------------------
-    def _evaluate(self, xyz: Tensor, want_phi: bool, want_force: bool, want_deriv: bool,
-                  t=0.0, out=None):
-        """out: optional preallocated (phi, force, deriv) tensors (None entries
-        are allocated)."""
+    # ---- primitives -----------------------------------------------------------
+    def _leaves(self, t=0.0):
+        """This potential at time t as fused-kernel leaves (see leaves.py)."""
         raise NotImplementedError
 
+    def _evaluate(self, xyz: Tensor, want_phi: bool, want_force: bool, want_deriv: bool,
+                  t=0.0, out=None):
+        """Phi / force / deriv at xyz; ``out``: optional preallocated
+        (phi, force, deriv) tensors (None entries are allocated).  Default: one
+        fused launch over this potential's leaves."""
+        from .leaves import eval_leaves
+        phi, force, deriv = alloc_outputs(xyz.shape[0], self.device, want_phi, want_force,
+                                          want_deriv, out)
+        eval_leaves(self._leaves(t), xyz, phi, force, deriv, self.device)
+        return phi, force, deriv
+
     def _density(self, xyz: Tensor, t=0.0) -> Tensor:
-        raise NotImplementedError
+        from .leaves import density_leaves
+        rho = torch.empty(xyz.shape[0], dtype=torch.float64, device=self.device)
+        density_leaves(self._leaves(t), xyz, rho, self.device)
+        return rho
 
     # ---- Agama API ----------------------------------------------------------
     def potential(self, xyz, t=0.0):

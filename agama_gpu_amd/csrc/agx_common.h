@@ -24,6 +24,9 @@ __device__ __forceinline__ int cldi(const int *p, int i)
 
 constexpr int kBlock = 256;
 
+// Agama's gravitational constant, kpc (km/s)^2 / Msol (reference gpu_potential.py:157)
+constexpr double G_AGAMA_D = 4.30091727067736e-06;
+
 // Grid size for a grid-stride launch over n items: enough blocks to fill
 // 256 CUs several times over, never more than needed.
 inline unsigned grid_for(int64_t n, int per_cu_blocks = 8)

@@ -1,0 +1,169 @@
+// Fused composite evaluation: any sum of analytic and expansion components,
+// each under its own shift/scale/amplitude transform, in ONE launch.
+//
+// The reference evaluates a Composite as one launch per component plus an
+// elementwise add per output (gpu_potential.py:1477-1510), a Shifted
+// potential as a host spline evaluation plus a subtract kernel
+// (gpu_potential.py:1720-1744), a Scaled one as two more elementwise kernels
+// (gpu_potential.py:1878-1900) and an Evolving one as two full evaluations and
+// a lerp (gpu_potential.py:1577-1644).  Here the wrapper tree is flattened on
+// the host into leaves, each with
+//     x_leaf = sigma (x - center),  Phi += A Phi_leaf(x_leaf),
+//     grad  += A sigma grad_leaf,   Hessian += A sigma^2 H_leaf,
+//     rho   += A sigma^2 rho_leaf
+// (A carries Scaled amplitudes, 1/scale factors and Evolving lerp weights),
+// and every particle runs through all leaves once: one read of the positions,
+// one write of each output.
+#include "analytic_dev.h"
+#include "multipole_dev.h"
+
+namespace agx {
+
+constexpr int kMaxComponents = AGX_MAX_COMPONENTS;
+
+struct CompositeArgs {
+    int n;
+    agx_component c[kMaxComponents];
+};
+
+// Read a device descriptor image with uniform (scalar) loads.
+template <typename T>
+__device__ __forceinline__ T load_desc(const void *src)
+{
+    static_assert(sizeof(T) % 4 == 0, "descriptor size");
+    T d;
+    int *dst = (int *)&d;
+#pragma unroll
+    for (int q = 0; q < (int)(sizeof(T) / 4); ++q) dst[q] = cldi((const int *)src, q);
+    return d;
+}
+
+template <int ORD>
+__global__ __launch_bounds__(kBlock) void composite_kernel(CompositeArgs A,
+                                                           const double *__restrict__ xyz,
+                                                           int64_t n, double *__restrict__ phi,
+                                                           double *__restrict__ force,
+                                                           double *__restrict__ deriv)
+{
+    constexpr bool G = ORD == ORD_GRAD || ORD == ORD_HESS;
+    constexpr bool H = ORD == ORD_HESS;
+    constexpr bool RHO = ORD == ORD_DENS;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        const double x0 = xyz[3 * i], y0 = xyz[3 * i + 1], z0 = xyz[3 * i + 2];
+        double P = 0.0, g[3] = {0.0, 0.0, 0.0}, h[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int c = 0; c < A.n; ++c) {
+            const agx_component &C = A.c[c];
+            const double sg = C.scale;
+            const double x = sg * (x0 - C.center[0]), y = sg * (y0 - C.center[1]),
+                         z = sg * (z0 - C.center[2]);
+            Eval3 e;
+            if (C.type == AGX_T_MULTIPOLE) {
+                const MPK mp = load_desc<MPK>(C.table);
+                PointOut o{};
+                Where w = classify(mp, x, y, z);
+                if (w.kind == 2)
+                    interior<ORD, false>(mp, mp.poly + (int64_t)w.k * mp.n_lm * 6, x, y, z, w.r2,
+                                         w.r, w.lnr, w.k, o, 1.0 / (4.0 * M_PI * G_AGAMA_D));
+                else
+                    extrap<ORD>(mp, w, x, y, z, o, 1.0 / (4.0 * M_PI * G_AGAMA_D));
+                if (RHO) e.rho = o.phi;
+                else e.phi = o.phi;
+                for (int k = 0; k < 3; ++k) e.g[k] = o.g[k];
+                for (int k = 0; k < 6; ++k) e.h[k] = o.h[k];
+            } else {
+                eval_analytic<G, H, RHO>(C.type, C.p, x, y, z, e);
+            }
+            const double a = C.ampl;
+            if (RHO) {
+                P = fma(a * sg * sg, e.rho, P);
+            } else {
+                P = fma(a, e.phi, P);
+                if (G) {
+                    const double ag = a * sg;
+                    for (int k = 0; k < 3; ++k) g[k] = fma(ag, e.g[k], g[k]);
+                }
+                if (H) {
+                    const double ah = a * sg * sg;
+                    for (int k = 0; k < 6; ++k) h[k] = fma(ah, e.h[k], h[k]);
+                }
+            }
+        }
+        if (phi) phi[i] = P;
+        if (G && force) { force[3 * i] = -g[0]; force[3 * i + 1] = -g[1]; force[3 * i + 2] = -g[2]; }
+        if (H && deriv)
+            for (int k = 0; k < 6; ++k) deriv[6 * i + k] = -h[k];
+    }
+}
+
+static int check_components(const agx_component *comps, int n_comps, CompositeArgs &A)
+{
+    if (n_comps < 1 || n_comps > kMaxComponents) return AGX_E_SIZE;
+    if (!comps) return AGX_E_NULL;
+    A.n = n_comps;
+    for (int c = 0; c < n_comps; ++c) {
+        const agx_component &C = comps[c];
+        const bool expansion = C.type == AGX_T_MULTIPOLE;
+        const bool analytic = C.type >= AGX_T_NFW && C.type <= AGX_T_UNIFORM_ACCEL;
+        if (!expansion && !analytic) return AGX_E_TYPE;
+        if (expansion && !C.table) return AGX_E_NULL;
+        A.c[c] = C;
+    }
+    return AGX_OK;
+}
+
+}  // namespace agx
+
+using namespace agx;
+
+extern "C" int agx_composite_eval(const agx_component *comps, int n_comps, const double *d_xyz,
+                                  int64_t n, double *d_phi, double *d_force, double *d_deriv,
+                                  uint32_t what, void *stream)
+{
+    CompositeArgs A;
+    int rc = check_components(comps, n_comps, A);
+    if (rc) return rc;
+    if (n < 0) return AGX_E_SIZE;
+    if (what == 0 || (what & ~(AGX_PHI | AGX_FORCE | AGX_DERIV))) return AGX_E_WHAT;
+    if (n == 0) return AGX_OK;
+    if (!d_xyz || ((what & AGX_PHI) && !d_phi) || ((what & AGX_FORCE) && !d_force) ||
+        ((what & AGX_DERIV) && !d_deriv))
+        return AGX_E_NULL;
+    double *phi = (what & AGX_PHI) ? d_phi : nullptr;
+    double *force = (what & AGX_FORCE) ? d_force : nullptr;
+    double *deriv = (what & AGX_DERIV) ? d_deriv : nullptr;
+    hipStream_t s = (hipStream_t)stream;
+    dim3 grid(grid_for(n)), block(kBlock);
+    if (deriv)
+        hipLaunchKernelGGL(composite_kernel<ORD_HESS>, grid, block, 0, s, A, d_xyz, n, phi, force, deriv);
+    else if (force)
+        hipLaunchKernelGGL(composite_kernel<ORD_GRAD>, grid, block, 0, s, A, d_xyz, n, phi, force, nullptr);
+    else
+        hipLaunchKernelGGL(composite_kernel<ORD_POT>, grid, block, 0, s, A, d_xyz, n, phi, nullptr, nullptr);
+    return hipGetLastError() == hipSuccess ? AGX_OK : AGX_E_LAUNCH;
+}
+
+extern "C" int agx_composite_density(const agx_component *comps, int n_comps, const double *d_xyz,
+                                     int64_t n, double *d_rho, void *stream)
+{
+    CompositeArgs A;
+    int rc = check_components(comps, n_comps, A);
+    if (rc) return rc;
+    if (n < 0) return AGX_E_SIZE;
+    if (n == 0) return AGX_OK;
+    if (!d_xyz || !d_rho) return AGX_E_NULL;
+    hipLaunchKernelGGL(composite_kernel<ORD_DENS>, dim3(grid_for(n)), dim3(kBlock), 0,
+                       (hipStream_t)stream, A, d_xyz, n, d_rho, nullptr, nullptr);
+    return hipGetLastError() == hipSuccess ? AGX_OK : AGX_E_LAUNCH;
+}
+
+extern "C" int64_t agx_multipole_pack_size(void) { return (int64_t)sizeof(MPK); }
+
+extern "C" int agx_multipole_pack(const agx_multipole *mp, void *dst, int64_t size)
+{
+    if (!mp || !dst) return AGX_E_NULL;
+    if (size < (int64_t)sizeof(MPK)) return AGX_E_WORKSPACE;
+    MPK P = make_args(mp);
+    __builtin_memcpy(dst, &P, sizeof(MPK));
+    return AGX_OK;
+}

@@ -1,354 +1,7 @@
-// Multipole (spherical-harmonic BFE) evaluation on gfx950, FP64.
-//
-// Same potential as the reference's multipole_*_kernel
-// (_multipole_potential_kernel.cu:140-958): quintic splines in ln r per
-// active (l,m) harmonic with Agama log-scaling, l=0 power/log-law
-// extrapolation inside r_min and beyond r_max, Agama-normalised associated
-// Legendre functions times cos/sin(|m| phi), Cartesian gradient and Hessian by
-// the chain rule.  The arithmetic is arranged for CDNA4 instead of following
-// the reference line by line:
-//
-//  * The Legendre recurrence runs in Agama's normalisation directly, with the
-//    per-(l,m) constants alpha_l, beta_l and the P_mm seed precomputed on the
-//    host (one FMA pair per step; no sqrt or division per step).
-//  * Terms are grouped by |m|: inside a group cos(m phi) and sin(m phi) are
-//    constants, so each term accumulates three products into per-group sums
-//    (V = sum C Q, R = sum C' Q, T = sum C dQ/dtheta) and the trigonometric
-//    factors are applied once per group.  The phi-derivative needs no extra
-//    per-term work: it is m(-sin V_cos + cos V_sin) per group.
-//  * With log-scaling every l>0 radial function is C_sc * Phi_0, so the
-//    monopole factor and its derivatives multiply the group sums once at the
-//    end instead of once per term.
-//  * Powers s^j and the derivative weights j s^(j-1) are formed once per
-//    particle; a term costs 5 FMA for C and 4 for C'.
-//  * Table layout is interval-major [k][c][6]: particles in one radial
-//    interval read one contiguous slab (scalar loads in the binned kernel).
-#include "agx_common.h"
+// Multipole kernels and C-ABI entry points (gfx950, FP64).
+#include "multipole_dev.h"
 
 namespace agx {
-
-struct MPK {
-    const double *poly;              // [n_int][n_lm][6]
-    const int *grp_m, *grp_step0, *step_cos, *step_sin, *step_l;
-    const double *qmm, *alpha, *beta;
-    int n_int, n_lm, n_groups, log_scaling, inner_log, outer_log;
-    double logr_min, dlogr, inv_dlogr, invPhi0;
-    double inner_s, inner_U, inner_W, outer_s, outer_U, outer_W;
-};
-
-enum { ORD_POT = 0, ORD_GRAD = 1, ORD_HESS = 2, ORD_DENS = 3 };
-
-struct PointOut {
-    double phi;
-    double g[3];   // +grad Phi
-    double h[6];   // +Hessian [xx, yy, zz, xy, yz, xz]
-};
-
-// l=0 extrapolation law and its first two ln r derivatives.
-//   log mode:   Phi = (W + U t) e^{v t}
-//   power mode: Phi = W e^{v t} + U e^{s t}
-__device__ __forceinline__ void law(double v, double s, double U, double W, int logm, double t,
-                                    double &P, double &D, double &D2)
-{
-    double ev = (v == 0.0) ? 1.0 : exp(v * t);
-    if (logm) {
-        double q = fma(U, t, W);
-        P = q * ev;
-        D = fma(v, q, U) * ev;
-        D2 = fma(v * v, q, 2.0 * v * U) * ev;
-    } else {
-        double es = (s != 0.0) ? exp(s * t) : 1.0;
-        double wv = W * ev, us = U * es;
-        P = wv + us;
-        D = fma(v, wv, s * us);
-        D2 = fma(v * v, wv, s * s * us);
-    }
-}
-
-// Radial/extrapolated part: fills `o` for points with a purely radial l=0
-// potential (inside r_min or beyond r_max).  dP, d2P are d/d ln r.
-template <int ORD>
-__device__ __forceinline__ void radial_out(double x, double y, double z, double r2,
-                                           double P, double D, double D2, PointOut &o)
-{
-    o.phi = P;
-    if (ORD == ORD_GRAD || ORD == ORD_HESS) {
-        double ir2 = 1.0 / r2, g = D * ir2;
-        o.g[0] = g * x; o.g[1] = g * y; o.g[2] = g * z;
-        if (ORD == ORD_HESS) {
-            // H_ij = (D2 - 2D) x_i x_j / r^4 + D delta_ij / r^2 for a radial Phi
-            double A = (D2 - 2.0 * D) * ir2 * ir2, B = g;
-            o.h[0] = fma(A * x, x, B); o.h[1] = fma(A * y, y, B); o.h[2] = fma(A * z, z, B);
-            o.h[3] = A * x * y; o.h[4] = A * y * z; o.h[5] = A * x * z;
-        }
-    }
-}
-
-// Interior evaluation on radial interval k (slab = poly + k*n_lm*6).
-// UNI: the slab address is wave-uniform (scalar loads of coefficients).
-template <int ORD, bool UNI>
-__device__ __forceinline__ void interior(const MPK &P, const double *slab, double x, double y,
-                                         double z, double r2, double r, double lnr, int k,
-                                         PointOut &o, double inv4piG)
-{
-    constexpr bool R1 = ORD != ORD_POT;                      // d/d ln r
-    constexpr bool D1 = ORD == ORD_GRAD || ORD == ORD_HESS;  // angular first derivatives
-    constexpr bool D2 = ORD == ORD_HESS || ORD == ORD_DENS;  // d^2/d ln r^2
-    constexpr bool ANG2 = ORD == ORD_HESS;                   // second angular derivatives
-    const double idl = P.inv_dlogr;
-    double s = (lnr - fma((double)k, P.dlogr, P.logr_min)) * idl;
-    s = fmin(fmax(s, 0.0), 1.0);
-    const double s2 = s * s, s3 = s2 * s, s4 = s2 * s2, s5 = s4 * s;
-    const double w2 = 2.0 * s, w3 = 3.0 * s2, w4 = 4.0 * s3, w5 = 5.0 * s4;     // C'
-    const double u3 = 6.0 * s, u4 = 12.0 * s2, u5 = 20.0 * s3;                  // C''
-
-    auto load6 = [&](int col, double a[6]) {
-        if (UNI) {
-            cdvec2_t *p = (cdvec2_t *)(slab + col * 6);
-            dvec2 q0 = p[0], q1 = p[1], q2 = p[2];
-            a[0] = q0.x; a[1] = q0.y; a[2] = q1.x; a[3] = q1.y; a[4] = q2.x; a[5] = q2.y;
-        } else {
-            const dvec2 *p = (const dvec2 *)(slab + col * 6);
-            dvec2 q0 = p[0], q1 = p[1], q2 = p[2];
-            a[0] = q0.x; a[1] = q0.y; a[2] = q1.x; a[3] = q1.y; a[4] = q2.x; a[5] = q2.y;
-        }
-    };
-    auto val = [&](const double a[6]) {
-        return fma(a[5], s5, fma(a[4], s4, fma(a[3], s3, fma(a[2], s2, fma(a[1], s, a[0])))));
-    };
-    auto der = [&](const double a[6]) {
-        return fma(a[5], w5, fma(a[4], w4, fma(a[3], w3, fma(a[2], w2, a[1]))));
-    };
-    auto der2 = [&](const double a[6]) {
-        return fma(a[5], u5, fma(a[4], u4, fma(a[3], u3, 2.0 * a[2])));
-    };
-
-    // ---- monopole with the log-scaling inverse ----------------------------
-    double a0[6];
-    load6(0, a0);
-    double C0 = val(a0), dC0 = 0.0, d2C0 = 0.0;
-    if (R1) dC0 = der(a0) * idl;
-    if (D2) d2C0 = der2(a0) * idl * idl;
-    double lam = 1.0, mu = 0.0, nu = 0.0;
-    if (P.log_scaling) {
-        double ex = exp(C0);
-        double P0 = 1.0 / (P.invPhi0 - ex);
-        double dPdX = P0 * P0 * ex;
-        if (D2) d2C0 = dPdX * fma(dC0 * dC0, P0 * (P.invPhi0 + ex), d2C0);
-        if (R1) dC0 = dPdX * dC0;
-        C0 = P0;
-        lam = C0; mu = dC0; nu = d2C0;
-    }
-
-    // ---- angular basis ------------------------------------------------------
-    const double R2 = fma(x, x, y * y);
-    const double R = sqrt(R2);
-    const double ir = 1.0 / r;
-    const double ct = z * ir, st = R * ir;
-    const bool pole = st < 1.0e-10;
-    double cph = 1.0, sph = 0.0;
-    if (R > 1.0e-300) { double iR = 1.0 / R; cph = x * iR; sph = y * iR; }
-    const double ist = pole ? 0.0 : 1.0 / st;
-
-    // scaled sums over all l>0 terms (trig applied)
-    double Sv = 0.0, Sr = 0.0, St = 0.0, Sfn = 0.0, Sfp = 0.0;
-    double Srr = 0.0, Stt = 0.0, Srt = 0.0, Sf = 0.0, Sff = 0.0, Srf = 0.0, Stf = 0.0, SL = 0.0;
-
-    double cm = 1.0, sm = 0.0, sp = 1.0, sp1 = 0.0;   // cos/sin(m phi), sin^m, sin^(m-1)
-    int cur_m = 0;
-    for (int g = 0; g < P.n_groups; ++g) {
-        const int m = cldi(P.grp_m, g);
-        while (cur_m < m) {
-            sp1 = sp; sp *= st;
-            double t = fma(cm, cph, -sm * sph);
-            sm = fma(sm, cph, cm * sph);
-            cm = t;
-            ++cur_m;
-        }
-        const double q = cld(P.qmm, g);
-        double Q1 = q * sp, dQ1 = 0.0, Q2 = 0.0, dQ2 = 0.0;
-        if (D1 && m > 0) dQ1 = (double)m * q * sp1 * ct;
-        double Vc = 0.0, Vs = 0.0, Rc = 0.0, Rs = 0.0, Tc = 0.0, Ts = 0.0;
-        double RRc = 0.0, RRs = 0.0, TTc = 0.0, TTs = 0.0, RTc = 0.0, RTs = 0.0, Lc = 0.0, Ls = 0.0;
-        const int j0 = cldi(P.grp_step0, g), j1 = cldi(P.grp_step0, g + 1);
-        for (int j = j0; j < j1; ++j) {
-            const int l = cldi(P.step_l, j);
-            if (l > m) {
-                const double al = cld(P.alpha, j), be = cld(P.beta, j);
-                double Qn = fma(al * ct, Q1, -be * Q2);
-                double dQn = 0.0;
-                if (D1) dQn = fma(al, fma(ct, dQ1, -st * Q1), -be * dQ2);
-                Q2 = Q1; Q1 = Qn;
-                dQ2 = dQ1; dQ1 = dQn;
-            }
-            double d2Q = 0.0;
-            if (ANG2 && !pole)
-                d2Q = -ct * ist * dQ1 - ((double)(l * (l + 1)) - (double)(m * m) * ist * ist) * Q1;
-            const double ll1 = (double)(l * (l + 1));
-            const int cc = cldi(P.step_cos, j), cs = cldi(P.step_sin, j);
-#define AGX_TERM(COL, V, Rr, Tt, RRr, TTt, RTt, LL)                           \
-            {                                                                 \
-                double a[6];                                                  \
-                load6(COL, a);                                                \
-                const double C = val(a);                                      \
-                V = fma(C, Q1, V);                                            \
-                if (R1) { const double dC = der(a);                           \
-                    Rr = fma(dC, Q1, Rr);                                     \
-                    if (D1) Tt = fma(C, dQ1, Tt);                             \
-                    if (ANG2) RTt = fma(dC, dQ1, RTt); }                      \
-                if (D2) { RRr = fma(der2(a), Q1, RRr); }                      \
-                if (ANG2) TTt = fma(C, d2Q, TTt);                             \
-                if (ORD == ORD_DENS) LL = fma(ll1 * C, Q1, LL);               \
-            }
-            if (cc >= 0) AGX_TERM(cc, Vc, Rc, Tc, RRc, TTc, RTc, Lc)
-            if (cs >= 0) AGX_TERM(cs, Vs, Rs, Ts, RRs, TTs, RTs, Ls)
-#undef AGX_TERM
-        }
-        // ---- apply cos/sin(m phi) to the group sums ------------------------
-        Sv = fma(cm, Vc, fma(sm, Vs, Sv));
-        if (R1) Sr = fma(cm, Rc, fma(sm, Rs, Sr));
-        if (D1) {
-            St = fma(cm, Tc, fma(sm, Ts, St));
-            if (m > 0) {
-                const double fm = (double)m;
-                const double Fv = fm * fma(cm, Vs, -sm * Vc);     // dPhi/dphi (group)
-                Sfn += Fv;
-                if (m == 1) Sfp += fma(cm, Ts, -sm * Tc);         // pole limit of Y/sin
-                if (ANG2) {
-                    Sf += Fv;
-                    Sff -= fm * fm * fma(cm, Vc, sm * Vs);
-                    Srf += fm * fma(cm, Rs, -sm * Rc);
-                    Stf += fm * fma(cm, Ts, -sm * Tc);
-                }
-            }
-        }
-        if (D2) Srr = fma(cm, RRc, fma(sm, RRs, Srr));
-        if (ANG2) {
-            Stt = fma(cm, TTc, fma(sm, TTs, Stt));
-            Srt = fma(cm, RTc, fma(sm, RTs, Srt));
-        }
-        if (ORD == ORD_DENS) SL = fma(cm, Lc, fma(sm, Ls, SL));
-    }
-
-    // ---- spherical derivatives of the full potential -------------------------
-    const double Pv = fma(lam, Sv, C0);
-    if (ORD == ORD_POT) { o.phi = Pv; return; }
-    const double Pr = fma(lam * idl, Sr, fma(mu, Sv, dC0));
-    if (ORD == ORD_DENS) {
-        const double Prr = fma(lam * idl * idl, Srr, fma(2.0 * mu * idl, Sr, fma(nu, Sv, d2C0)));
-        o.phi = inv4piG * (Prr + Pr - lam * SL) / r2;
-        return;
-    }
-    o.phi = Pv;
-    const double Pt = lam * St;
-    const double Pfos = lam * (pole ? Sfp : Sfn * ist);
-    const double ir2 = ir * ir;
-    o.g[0] = fma(Pr * x, ir2, fma(Pt, ct * cph * ir, -Pfos * sph * ir));
-    o.g[1] = fma(Pr * y, ir2, fma(Pt, ct * sph * ir, Pfos * cph * ir));
-    o.g[2] = fma(Pr * z, ir2, -Pt * st * ir);
-    if (ORD != ORD_HESS) return;
-
-    const double Prr = fma(lam * idl * idl, Srr, fma(2.0 * mu * idl, Sr, fma(nu, Sv, d2C0)));
-    const double Ptt = lam * Stt;
-    const double Prt = fma(lam * idl, Srt, mu * St);
-    const double Pf = lam * Sf, Pff = lam * Sff, Ptf = lam * Stf;
-    const double Prf = fma(lam * idl, Srf, mu * Sf);
-    // first derivatives of (ln r, theta, phi); pole-regularised as the reference
-    const double iR = (R > 1.0e-300) ? 1.0 / R : 0.0;
-    const double c3[3] = {x, y, z};
-    const double dl[3] = {x * ir2, y * ir2, z * ir2};
-    const double dt[3] = {ct * cph * ir, ct * sph * ir, -st * ir};
-    const double dp[3] = {-sph * iR, cph * iR, 0.0};
-    const double ir3 = ir2 * ir, ir4 = ir2 * ir2, ir5 = ir4 * ir;
-    const double dct[3] = {-z * x * ir3, -z * y * ir3, ir - z * z * ir3};   // d cos(theta)
-    const double isth = (st > 1.0e-14) ? 1.0 / st : 0.0;
-    const double cos3 = ct * isth * isth * isth;
-    double iR4 = 0.0;
-    if (R > 1.0e-14) iR4 = iR * iR * iR * iR;
-    const int ia[6] = {0, 1, 2, 0, 1, 0}, ib[6] = {0, 1, 2, 1, 2, 2};
-#pragma unroll
-    for (int p = 0; p < 6; ++p) {
-        const int a = ia[p], b = ib[p];
-        const double dab = (a == b) ? 1.0 : 0.0;
-        const double d2lr = dab * ir2 - 2.0 * c3[a] * c3[b] * ir4;
-        const double d2ct = -(dab * z + (a == 2 ? c3[b] : 0.0) + (b == 2 ? c3[a] : 0.0)) * ir3
-                            + 3.0 * z * c3[a] * c3[b] * ir5;
-        const double d2th = -d2ct * isth - dct[a] * dct[b] * cos3;
-        double d2ph = 0.0;
-        if (a == 0 && b == 0) d2ph = 2.0 * x * y * iR4;
-        else if (a == 1 && b == 1) d2ph = -2.0 * x * y * iR4;
-        else if (a == 0 && b == 1) d2ph = (y * y - x * x) * iR4;
-        o.h[p] = Prr * dl[a] * dl[b]
-               + Prt * (dl[a] * dt[b] + dl[b] * dt[a])
-               + Prf * (dl[a] * dp[b] + dl[b] * dp[a])
-               + Ptt * dt[a] * dt[b]
-               + Ptf * (dt[a] * dp[b] + dt[b] * dp[a])
-               + Pff * dp[a] * dp[b]
-               + Pr * d2lr + Pt * d2th + Pf * d2ph;
-    }
-}
-
-// Classification of one point: 0 origin, 1 inner law, 2 interior (k), 3 outer law.
-struct Where {
-    int kind, k;
-    double r2, r, lnr;
-};
-
-__device__ __forceinline__ Where classify(const MPK &P, double x, double y, double z)
-{
-    Where w;
-    w.r2 = fma(x, x, fma(y, y, z * z));
-    w.r = sqrt(w.r2);
-    w.k = 0;
-    w.lnr = 0.0;
-    if (!(w.r >= 1.0e-300)) { w.kind = 0; return w; }
-    w.lnr = log(w.r);
-    if (w.lnr < P.logr_min) { w.kind = 1; return w; }
-    int k = (int)((w.lnr - P.logr_min) * P.inv_dlogr);
-    if (k < 0) k = 0;
-    if (k >= P.n_int) { w.kind = 3; return w; }
-    w.kind = 2;
-    w.k = k;
-    return w;
-}
-
-template <int ORD>
-__device__ __forceinline__ void extrap(const MPK &P, const Where &w, double x, double y, double z,
-                                       PointOut &o, double inv4piG)
-{
-    double Pv, D, D2;
-    if (w.kind == 0) {      // origin: Phi = W, zero force and Hessian (reference :166-171)
-        o.phi = (ORD == ORD_DENS) ? 0.0 : P.inner_W;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) o.g[a] = 0.0;
-#pragma unroll
-        for (int a = 0; a < 6; ++a) o.h[a] = 0.0;
-        return;
-    }
-    if (w.kind == 1) {
-        law(0.0, P.inner_s, P.inner_U, P.inner_W, P.inner_log, w.lnr - P.logr_min, Pv, D, D2);
-    } else {
-        if (ORD == ORD_DENS) { o.phi = 0.0; return; }
-        law(-1.0, P.outer_s, P.outer_U, P.outer_W, P.outer_log,
-            w.lnr - fma((double)P.n_int, P.dlogr, P.logr_min), Pv, D, D2);
-    }
-    if (ORD == ORD_DENS) { o.phi = inv4piG * (D2 + D) / w.r2; return; }
-    radial_out<ORD>(x, y, z, w.r2, Pv, D, D2, o);
-}
-
-template <int ORD>
-__device__ __forceinline__ void store(const PointOut &o, int64_t i, double *phi, double *force,
-                                      double *deriv)
-{
-    if (phi) phi[i] = o.phi;
-    if ((ORD == ORD_GRAD || ORD == ORD_HESS) && force) {
-        force[3 * i] = -o.g[0]; force[3 * i + 1] = -o.g[1]; force[3 * i + 2] = -o.g[2];
-    }
-    if (ORD == ORD_HESS && deriv) {
-#pragma unroll
-        for (int p = 0; p < 6; ++p) deriv[6 * i + p] = -o.h[p];
-    }
-}
 
 // One particle per lane, grid-stride over N.
 template <int ORD>
@@ -458,28 +111,6 @@ __global__ __launch_bounds__(kBinThreads) void mp_binned_kernel(
         }
         __syncthreads();
     }
-}
-
-static MPK make_args(const agx_multipole *mp)
-{
-    MPK P;
-    const int G = mp->n_groups, S = mp->n_steps;
-    P.poly = mp->poly_km;
-    P.grp_m = mp->prog_i;
-    P.grp_step0 = mp->prog_i + G;
-    P.step_cos = mp->prog_i + 2 * G + 1;
-    P.step_sin = P.step_cos + S;
-    P.step_l = P.step_sin + S;
-    P.qmm = mp->prog_d;
-    P.alpha = mp->prog_d + G;
-    P.beta = P.alpha + S;
-    P.n_int = mp->n_int; P.n_lm = mp->n_lm; P.n_groups = G;
-    P.log_scaling = mp->log_scaling; P.inner_log = mp->inner_log; P.outer_log = mp->outer_log;
-    P.logr_min = mp->logr_min; P.dlogr = mp->dlogr; P.inv_dlogr = mp->inv_dlogr;
-    P.invPhi0 = mp->invPhi0;
-    P.inner_s = mp->inner_s; P.inner_U = mp->inner_U; P.inner_W = mp->inner_W;
-    P.outer_s = mp->outer_s; P.outer_U = mp->outer_U; P.outer_W = mp->outer_W;
-    return P;
 }
 
 static int check_mp(const agx_multipole *mp, const double *xyz, int64_t n)

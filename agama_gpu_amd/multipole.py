@@ -54,6 +54,12 @@ class MultipolePotentialGPU(PotentialBase):
                   "outer_s", "outer_U", "outer_W"):
             setattr(s, k, d[k])
         self._desc = s
+        # device image of the descriptor, for fused composites
+        L = _lib.load()
+        nbytes = int(L.agx_multipole_pack_size())
+        img = (C.c_uint8 * nbytes)()
+        _lib.check(L.agx_multipole_pack(C.byref(s), img, nbytes), "agx_multipole_pack")
+        self._dev_desc = torch.frombuffer(bytearray(img), dtype=torch.uint8).to(dev)
         self._inv_4piG = INV_4PIG
         self.lmax = d["lmax"]
         self.n_lm = d["n_lm"]
@@ -77,6 +83,10 @@ class MultipolePotentialGPU(PotentialBase):
         return self._desc
 
     # ---- primitives ---------------------------------------------------------
+    def _leaves(self, t=0.0):
+        from .leaves import Leaf
+        return [Leaf(_lib.AGX_T_MULTIPOLE, table=self._dev_desc.data_ptr(), keepalive=self)]
+
     def _evaluate(self, xyz, want_phi, want_force, want_deriv, t=0.0, out=None):
         n = xyz.shape[0]
         phi, force, deriv = alloc_outputs(n, self.device, want_phi, want_force, want_deriv, out)

@@ -82,6 +82,57 @@ int agx_multipole_eval(const agx_multipole *mp, const double *d_xyz, int64_t n,
 int agx_multipole_density(const agx_multipole *mp, const double *d_xyz, int64_t n,
                           double inv_4piG, double *d_rho, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Fused composite: a sum of components, each evaluated at
+ *     x_leaf = scale * (x - center)
+ * and accumulated as  Phi += ampl * Phi_leaf,  force += ampl * scale * F_leaf,
+ * deriv += ampl * scale^2 * D_leaf,  rho += ampl * scale^2 * rho_leaf,
+ * in one launch.  Composite, Shifted, Scaled and Evolving wrappers flatten into
+ * such leaves (Evolving lerp weights go into ampl).
+ * Replaces the reference's per-component launches and elementwise glue of
+ * CompositePotentialGPU (gpu_potential.py:1439-1529), EvolvingPotentialGPU
+ * (1536-1644), ShiftedPotentialGPU (1651-1764) and ScaledPotentialGPU
+ * (1767-1918), and the analytic CuPy kernels (_analytic_potentials.py:176-1165).
+ *
+ * Analytic parameters p[] per type:
+ *   NFW            {G M, r_s, M}           Plummer   {G M, b, M}
+ *   Hernquist      {G M, a, M}             Dehnen    {G M, a, gamma, M}
+ *   Isochrone      {G M, b, M}             MiyamotoNagai {G M, a, b, M}
+ *   LogHalo        {v0^2, r_c^2, p^2, q^2} DiskAnsatz {4 pi G Sigma, R_d, h, r_in}
+ *   UniformAcceleration {a_x, a_y, a_z}
+ * Expansion types carry `table`: the device address of a descriptor image
+ * written by agx_multipole_pack and copied to device memory.
+ * ---------------------------------------------------------------------- */
+enum {
+    AGX_T_NFW = 1, AGX_T_PLUMMER = 2, AGX_T_HERNQUIST = 3, AGX_T_DEHNEN = 4,
+    AGX_T_ISOCHRONE = 5, AGX_T_MIYAMOTO_NAGAI = 6, AGX_T_LOGHALO = 7,
+    AGX_T_DISK_ANSATZ = 8, AGX_T_UNIFORM_ACCEL = 9,
+    AGX_T_MULTIPOLE = 16, AGX_T_CYLSPLINE = 17
+};
+#define AGX_MAX_COMPONENTS 16
+
+typedef struct agx_component {
+    int32_t type;
+    int32_t pad;
+    double p[8];
+    const void *table;
+    double ampl;
+    double scale;
+    double center[3];
+} agx_component;
+
+int agx_composite_eval(const agx_component *comps, int n_comps, const double *d_xyz, int64_t n,
+                       double *d_phi, double *d_force, double *d_deriv, uint32_t what,
+                       void *stream);
+int agx_composite_density(const agx_component *comps, int n_comps, const double *d_xyz,
+                          int64_t n, double *d_rho, void *stream);
+
+/* Device descriptor image of a Multipole table for agx_component.table:
+ * agx_multipole_pack writes agx_multipole_pack_size() bytes into the host
+ * buffer `dst`; the caller copies them to device memory. */
+int64_t agx_multipole_pack_size(void);
+int agx_multipole_pack(const agx_multipole *mp, void *dst, int64_t size);
+
 /* Tuning options (process-wide; set before launches, not thread-safe):
  *   AGX_OPT_MP_PATH      Multipole evaluation path: 0 automatic, 1 one particle
  *                        per lane with per-lane table reads, 2 radially binned
