@@ -10,6 +10,7 @@ from .coefs import CylSplineCoefs, MultipoleCoefs, read_coefs, write_coefs
 from .multipole_prep import G_AGAMA, INV_4PIG
 from .base import PotentialBase
 from .multipole import MultipolePotentialGPU
+from .cylspline import CylSplinePotentialGPU
 from .composite import (CompositePotentialGPU, EvolvingPotentialGPU, ScaledPotentialGPU,
                         ShiftedPotentialGPU)
 from .analytic import (AnalyticPotentialGPU, DehnenSphericalPotentialGPU, DiskAnsatzPotentialGPU,
@@ -19,7 +20,8 @@ from .analytic import (AnalyticPotentialGPU, DehnenSphericalPotentialGPU, DiskAn
 from .factory import PotentialGPU
 
 __all__ = [
-    "PotentialGPU", "PotentialBase", "MultipolePotentialGPU", "CompositePotentialGPU",
+    "PotentialGPU", "PotentialBase", "MultipolePotentialGPU", "CylSplinePotentialGPU",
+    "CompositePotentialGPU",
     "EvolvingPotentialGPU", "ShiftedPotentialGPU", "ScaledPotentialGPU",
     "AnalyticPotentialGPU", "NFWPotentialGPU", "PlummerPotentialGPU", "HernquistPotentialGPU",
     "DehnenSphericalPotentialGPU", "IsochronePotentialGPU", "MiyamotoNagaiPotentialGPU",

@@ -36,6 +36,16 @@ class agx_multipole(C.Structure):
     ]
 
 
+class agx_cylspline(C.Structure):
+    _fields_ = [
+        ("lR_grid", C.c_void_p), ("lz_grid", C.c_void_p), ("nodes", C.c_void_p),
+        ("m_arr", C.c_void_p), ("W_outer", C.c_void_p),
+        ("nR", C.c_int32), ("nz", C.c_int32), ("n_harm", C.c_int32), ("log_scaling", C.c_int32),
+        ("lmax_outer", C.c_int32), ("mmax_outer", C.c_int32),
+        ("Rscale", C.c_double), ("r0_outer", C.c_double),
+    ]
+
+
 class agx_component(C.Structure):
     _fields_ = [
         ("type", C.c_int32), ("pad", C.c_int32), ("p", C.c_double * 8),
@@ -72,6 +82,12 @@ _SIGNATURES = {
                                      C.c_void_p]),
     "agx_composite_density": (C.c_int, [C.POINTER(agx_component), C.c_int, C.c_void_p,
                                         C.c_int64, C.c_void_p, C.c_void_p]),
+    "agx_cylspline_eval": (C.c_int, [C.POINTER(agx_cylspline), C.c_void_p, C.c_int64,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "agx_cylspline_density": (C.c_int, [C.POINTER(agx_cylspline), C.c_void_p, C.c_int64,
+                                        C.c_void_p, C.c_void_p]),
+    "agx_cylspline_pack_size": (C.c_int64, []),
+    "agx_cylspline_pack": (C.c_int, [C.POINTER(agx_cylspline), C.c_void_p, C.c_int64]),
     "agx_multipole_pack_size": (C.c_int64, []),
     "agx_multipole_pack": (C.c_int, [C.POINTER(agx_multipole), C.c_void_p, C.c_int64]),
 }

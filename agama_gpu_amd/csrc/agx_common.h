@@ -24,6 +24,16 @@ __device__ __forceinline__ int cldi(const int *p, int i)
 
 constexpr int kBlock = 256;
 
+// Derivative order of an evaluation: potential; +gradient; +Hessian; density.
+enum { ORD_POT = 0, ORD_GRAD = 1, ORD_HESS = 2, ORD_DENS = 3 };
+
+// One evaluated point (density is returned in phi for ORD_DENS).
+struct PointOut {
+    double phi;
+    double g[3];   // +grad Phi
+    double h[6];   // +Hessian [xx, yy, zz, xy, yz, xz]
+};
+
 // Agama's gravitational constant, kpc (km/s)^2 / Msol (reference gpu_potential.py:157)
 constexpr double G_AGAMA_D = 4.30091727067736e-06;
 

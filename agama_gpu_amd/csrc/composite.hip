@@ -15,6 +15,7 @@
 // and every particle runs through all leaves once: one read of the positions,
 // one write of each output.
 #include "analytic_dev.h"
+#include "cylspline_dev.h"
 #include "multipole_dev.h"
 
 namespace agx {
@@ -71,6 +72,14 @@ __global__ __launch_bounds__(kBlock) void composite_kernel(CompositeArgs A,
                 else e.phi = o.phi;
                 for (int k = 0; k < 3; ++k) e.g[k] = o.g[k];
                 for (int k = 0; k < 6; ++k) e.h[k] = o.h[k];
+            } else if (C.type == AGX_T_CYLSPLINE) {
+                const CSK cs = load_desc<CSK>(C.table);
+                PointOut o{};
+                cylspline_point<ORD>(cs, x, y, z, o);
+                if (RHO) e.rho = o.phi;
+                else e.phi = o.phi;
+                for (int k = 0; k < 3; ++k) e.g[k] = o.g[k];
+                for (int k = 0; k < 6; ++k) e.h[k] = o.h[k];
             } else {
                 eval_analytic<G, H, RHO>(C.type, C.p, x, y, z, e);
             }
@@ -103,7 +112,7 @@ static int check_components(const agx_component *comps, int n_comps, CompositeAr
     A.n = n_comps;
     for (int c = 0; c < n_comps; ++c) {
         const agx_component &C = comps[c];
-        const bool expansion = C.type == AGX_T_MULTIPOLE;
+        const bool expansion = C.type == AGX_T_MULTIPOLE || C.type == AGX_T_CYLSPLINE;
         const bool analytic = C.type >= AGX_T_NFW && C.type <= AGX_T_UNIFORM_ACCEL;
         if (!expansion && !analytic) return AGX_E_TYPE;
         if (expansion && !C.table) return AGX_E_NULL;
@@ -158,6 +167,17 @@ extern "C" int agx_composite_density(const agx_component *comps, int n_comps, co
 }
 
 extern "C" int64_t agx_multipole_pack_size(void) { return (int64_t)sizeof(MPK); }
+
+extern "C" int64_t agx_cylspline_pack_size(void) { return (int64_t)sizeof(CSK); }
+
+extern "C" int agx_cylspline_pack(const agx_cylspline *cs, void *dst, int64_t size)
+{
+    if (!cs || !dst) return AGX_E_NULL;
+    if (size < (int64_t)sizeof(CSK)) return AGX_E_WORKSPACE;
+    CSK P = make_cyl_args(cs);
+    __builtin_memcpy(dst, &P, sizeof(CSK));
+    return AGX_OK;
+}
 
 extern "C" int agx_multipole_pack(const agx_multipole *mp, void *dst, int64_t size)
 {

@@ -1,0 +1,304 @@
+// CylSpline (azimuthal-harmonic bicubic spline) device code for gfx950, FP64.
+//
+// Same potential as the reference's cylspl_eval_device
+// (_cylspl_potential_kernel.cu:396-863): per azimuthal harmonic m a cubic
+// Hermite spline in the asinh-scaled coordinates lR = asinh(R/Rscale),
+// lz = asinh(z/Rscale) from node values and derivatives (f, f_lR, f_lz,
+// f_lR lz), log-scaling of m=0 (Phi0 = -exp(f0), others Phi_m/Phi0), the
+// Fourier sum in phi, and outside the grid a PowerLaw multipole
+// Phi = sum_lm W_lm (r0/r)^(l+1) mul Ptilde_lm T_m.  Differences from the
+// reference, each a correction:
+//  * the Cartesian Hessian includes the dPhi/dphi terms of the
+//    cylindrical-to-Cartesian chain rule (the reference drops them, :820-846);
+//  * outside the grid the gradient includes dPhi/dtheta and the Hessian all
+//    harmonics (the reference keeps only dPhi/dr, dPhi/dphi and a monopole
+//    Hessian, :354-382);
+//  * the associated Legendre recurrence uses P_(l-2) (the reference's
+//    sphHarm_device feeds 0 / P_(l-3) there, :205-223).
+#pragma once
+#include "sph_dev.h"
+
+namespace agx {
+
+struct CSK {
+    const double *lR, *lz;        // [nR], [nz] asinh-scaled grids
+    const double *nodes;          // [n_harm][nR][nz][4]
+    const int *m_arr;             // [n_harm]
+    const double *W;              // [(lmax_o+1)^2] outer PowerLaw, index l(l+1)+m
+    int nR, nz, n_harm, log_scaling, lmax_o, mmax_o;
+    double Rscale, r0;
+};
+
+// cell index i with grid[i] <= x < grid[i+1], clamped to [0, n-2]
+__device__ __forceinline__ int cell_search(double x, const double *g, int n)
+{
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (cld(g, mid) > x) hi = mid; else lo = mid;
+    }
+    return lo;
+}
+
+// cubic Hermite on [xl, xh] from values fl, fh and slopes dl, dh
+// (the reference's evalCubic1D, :110-128)
+template <bool D1, bool D2>
+__device__ __forceinline__ void herm1(double t, double h, double fl, double fh, double dl, double dh,
+                                      double &f, double &df, double &d2f)
+{
+    const double T = 1.0 - t, tT = t * T, dif = fh - fl;
+    const double Q = 3.0 * (dl + dh) - 6.0 * dif / h;
+    f = fl * T + fh * t + (dif * (t - T) + (dl * T - dh * t) * h) * tT;
+    if (D1) df = dl * T + dh * t - Q * tT;
+    if (D2) d2f = (dh - dl + Q * (t - T)) / h;
+}
+
+// Outer PowerLaw multipole (lmax_o <= 8): spherical derivatives.
+template <bool G, bool H, bool RHO>
+__device__ __forceinline__ void cyl_outer(const CSK &P, double x, double y, double z, double R,
+                                          PointOut &o)
+{
+    const double r2 = fma(R, R, z * z), r = sqrt(r2), ir = 1.0 / r;
+    if (RHO) { o.phi = 0.0; return; }   // a harmonic expansion: zero density
+    const double ct = z * ir, st = R * ir;
+    const bool pole = st < 1.0e-10;
+    double cph = 1.0, sph = 0.0;
+    if (R > 1.0e-300) { cph = x / R; sph = y / R; }
+    const double ist = pole ? 0.0 : 1.0 / st;
+    const double q = P.r0 * ir;
+    double Sv = 0, Sr = 0, Srr = 0, St = 0, Stt = 0, Srt = 0, Sf = 0, Sff = 0, Srf = 0, Stf = 0;
+    double Sfn = 0, Sfp = 0;
+    double cm = 1.0, sm = 0.0, pmm = 1.0 / sqrt(4.0 * M_PI), dpmm = 0.0;
+    for (int m = 0; m <= P.mmax_o; ++m) {
+        if (m > 0) {
+            const double t = fma(cm, cph, -sm * sph);
+            sm = fma(sm, cph, cm * sph); cm = t;
+            const double f = -sqrt((2.0 * m + 1.0) / (2.0 * m));
+            dpmm = f * fma(ct, pmm, st * dpmm);   // d/dtheta of f sin(theta) P_(m-1,m-1)
+            pmm = f * st * pmm;
+        }
+        const double mul = (m == 0) ? 2.0 * sqrt(M_PI) : 2.0 * sqrt(2.0 * M_PI);
+        double Q1 = pmm, dQ1 = dpmm, Q2 = 0.0, dQ2 = 0.0;
+        double rl = q;
+        for (int l = 0; l < m; ++l) rl *= q;      // (r0/r)^(m+1)
+        double Vc = 0, Vs = 0, Rc = 0, Rs = 0, RRc = 0, RRs = 0, Tc = 0, Ts = 0;
+        double TTc = 0, TTs = 0, RTc = 0, RTs = 0;
+        for (int l = m; l <= P.lmax_o; ++l) {
+            if (l > m) {
+                double a, b;
+                if (l == m + 1) { a = sqrt(2.0 * m + 3.0); b = 0.0; }
+                else {
+                    const double l2 = (double)l * l, m2 = (double)m * m;
+                    a = sqrt((4.0 * l2 - 1.0) / (l2 - m2));
+                    b = sqrt((2.0 * l + 1.0) * ((l - 1.0) * (l - 1.0) - m2) / ((2.0 * l - 3.0) * (l2 - m2)));
+                }
+                const double Qn = fma(a * ct, Q1, -b * Q2);
+                const double dQn = fma(a, fma(ct, dQ1, -st * Q1), -b * dQ2);
+                Q2 = Q1; Q1 = Qn; dQ2 = dQ1; dQ1 = dQn;
+                rl *= q;
+            }
+            const double Y = mul * Q1, dY = mul * dQ1;
+            const double d2Y = pole ? 0.0
+                : -ct * ist * dY - ((double)(l * (l + 1)) - (double)(m * m) * ist * ist) * Y;
+            const double wc = cld(P.W, l * (l + 1) + m) * rl;
+            const double ws = (m > 0) ? cld(P.W, l * (l + 1) - m) * rl : 0.0;
+            const double k1 = -(double)(l + 1), k2 = k1 * k1;   // d/dlnr, d2/dlnr2 of (r0/r)^(l+1)
+            Vc = fma(wc, Y, Vc); Vs = fma(ws, Y, Vs);
+            Rc = fma(k1 * wc, Y, Rc); Rs = fma(k1 * ws, Y, Rs);
+            Tc = fma(wc, dY, Tc); Ts = fma(ws, dY, Ts);
+            if (H) {
+                RRc = fma(k2 * wc, Y, RRc); RRs = fma(k2 * ws, Y, RRs);
+                TTc = fma(wc, d2Y, TTc); TTs = fma(ws, d2Y, TTs);
+                RTc = fma(k1 * wc, dY, RTc); RTs = fma(k1 * ws, dY, RTs);
+            }
+        }
+        Sv = fma(cm, Vc, fma(sm, Vs, Sv));
+        Sr = fma(cm, Rc, fma(sm, Rs, Sr));
+        St = fma(cm, Tc, fma(sm, Ts, St));
+        if (m > 0) {
+            const double fm = (double)m, Fv = fm * fma(cm, Vs, -sm * Vc);
+            Sfn += Fv;
+            if (m == 1) Sfp += fma(cm, Ts, -sm * Tc);
+            if (H) {
+                Sf += Fv;
+                Sff -= fm * fm * fma(cm, Vc, sm * Vs);
+                Srf += fm * fma(cm, Rs, -sm * Rc);
+                Stf += fm * fma(cm, Ts, -sm * Tc);
+            }
+        }
+        if (H) {
+            Srr = fma(cm, RRc, fma(sm, RRs, Srr));
+            Stt = fma(cm, TTc, fma(sm, TTs, Stt));
+            Srt = fma(cm, RTc, fma(sm, RTs, Srt));
+        }
+    }
+    o.phi = Sv;
+    if (!G && !H) return;
+    SphDerivs d;
+    d.Pr = Sr; d.Pt = St; d.Pfos = pole ? Sfp : Sfn * ist;
+    d.Prr = Srr; d.Prt = Srt; d.Prf = Srf; d.Ptt = Stt; d.Ptf = Stf; d.Pf = Sf; d.Pff = Sff;
+    sph_to_cart<H>(x, y, z, ir, R, ct, st, cph, sph, d, o.g, o.h);
+}
+
+// One point of a CylSpline potential.  ORD as the Multipole kernels.
+template <int ORD>
+__device__ __forceinline__ void cylspline_point(const CSK &P, double x, double y, double z,
+                                                PointOut &o)
+{
+    constexpr bool G = ORD == ORD_GRAD || ORD == ORD_HESS;
+    constexpr bool H = ORD == ORD_HESS || ORD == ORD_DENS;   // density from the trace
+    const double R2 = fma(x, x, y * y), R = sqrt(R2);
+    const double iRs = 1.0 / P.Rscale;
+    const double lR = asinh(R * iRs), lz = asinh(z * iRs);
+    if (lR < cld(P.lR, 0) || lR > cld(P.lR, P.nR - 1) || lz < cld(P.lz, 0) ||
+        lz > cld(P.lz, P.nz - 1)) {
+        cyl_outer<G || H, H, ORD == ORD_DENS>(P, x, y, z, R, o);
+        return;
+    }
+    const int iR = cell_search(lR, P.lR, P.nR), iz = cell_search(lz, P.lz, P.nz);
+    const double lR0 = cld(P.lR, iR), lR1 = cld(P.lR, iR + 1);
+    const double lz0 = cld(P.lz, iz), lz1 = cld(P.lz, iz + 1);
+    const double hR = lR1 - lR0, hz = lz1 - lz0;
+    const double tR = (lR - lR0) / hR, tz = (lz - lz0) / hz;
+    double cph = 1.0, sph = 0.0;
+    if (R > 1.0e-300) { cph = x / R; sph = y / R; }
+
+    // Fourier sums of the (scaled) harmonics and their lR, lz, phi derivatives
+    double S = P.log_scaling ? 1.0 : 0.0, SR = 0, SZ = 0, SRR = 0, SRZ = 0, SZZ = 0;
+    double Sp = 0, Spp = 0, SRp = 0, SZp = 0;
+    double F0 = 0, F0R = 0, F0Z = 0, F0RR = 0, F0RZ = 0, F0ZZ = 0;
+    int cur_m = 0;
+    double cm = 1.0, sm = 0.0;
+    const int64_t plane = (int64_t)P.nR * P.nz;
+    for (int mm = 0; mm < P.n_harm; ++mm) {
+        const int mv = cldi(P.m_arr, mm), am = mv < 0 ? -mv : mv;
+        // corners (iR,iz), (iR,iz+1) and (iR+1,iz), (iR+1,iz+1): two 64-byte runs
+        const dvec2 *lo = (const dvec2 *)(P.nodes + ((int64_t)mm * plane + (int64_t)iR * P.nz + iz) * 4);
+        const dvec2 *hi = lo + 2 * P.nz;
+        const dvec2 a0 = lo[0], a1 = lo[1], b0 = lo[2], b1 = lo[3];   // (iR,iz): f,fx | fy,fxy ; (iR,iz+1)
+        const dvec2 c0 = hi[0], c1 = hi[1], e0 = hi[2], e1 = hi[3];   // (iR+1,iz) ; (iR+1,iz+1)
+        // offset by one corner value for accuracy (reference :551-554)
+        const double off = (lR == lR1) ? ((lz == lz1) ? e0.x : c0.x) : ((lz == lz1) ? b0.x : a0.x);
+        // step 1: splines in lz of (f, f at iR+1, fx, fx at iR+1)
+        double Fz[4], dFz[4], d2Fz[4];
+        const double fl[4] = {a0.x - off, c0.x - off, a0.y, c0.y};
+        const double fu[4] = {b0.x - off, e0.x - off, b0.y, e0.y};
+        const double gl[4] = {a1.x, c1.x, a1.y, c1.y};
+        const double gu[4] = {b1.x, e1.x, b1.y, e1.y};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            herm1<G || H, H>(tz, hz, fl[k], fu[k], gl[k], gu[k], Fz[k], dFz[k], d2Fz[k]);
+        // step 2: splines in lR
+        double v, vR = 0, vRR = 0, vZ = 0, vRZ = 0, vZZ = 0, dummy;
+        herm1<G || H, H>(tR, hR, Fz[0], Fz[1], Fz[2], Fz[3], v, vR, vRR);
+        v += off;
+        if (G || H) herm1<H, false>(tR, hR, dFz[0], dFz[1], dFz[2], dFz[3], vZ, vRZ, dummy);
+        if (H) herm1<false, false>(tR, hR, d2Fz[0], d2Fz[1], d2Fz[2], d2Fz[3], vZZ, dummy, dummy);
+        if (mv == 0) {
+            F0 = v; F0R = vR; F0Z = vZ; F0RR = vRR; F0RZ = vRZ; F0ZZ = vZZ;
+            if (!P.log_scaling) {
+                S += v; SR += vR; SZ += vZ; SRR += vRR; SRZ += vRZ; SZZ += vZZ;
+            }
+            continue;
+        }
+        while (cur_m < am) {                       // cos/sin(|m| phi) by rotation
+            const double t = fma(cm, cph, -sm * sph);
+            sm = fma(sm, cph, cm * sph); cm = t; ++cur_m;
+        }
+        if (cur_m > am) {                          // harmonics are sorted by m, |m| may drop
+            cm = 1.0; sm = 0.0; cur_m = 0;
+            while (cur_m < am) {
+                const double t = fma(cm, cph, -sm * sph);
+                sm = fma(sm, cph, cm * sph); cm = t; ++cur_m;
+            }
+        }
+        const double T = (mv > 0) ? cm : sm;
+        const double dT = (mv > 0) ? -(double)am * sm : (double)am * cm;
+        const double d2T = -(double)(am * am) * T;
+        S = fma(v, T, S); SR = fma(vR, T, SR); SZ = fma(vZ, T, SZ);
+        Sp = fma(v, dT, Sp);
+        if (H) {
+            SRR = fma(vRR, T, SRR); SRZ = fma(vRZ, T, SRZ); SZZ = fma(vZZ, T, SZZ);
+            SRp = fma(vR, dT, SRp); SZp = fma(vZ, dT, SZp); Spp = fma(v, d2T, Spp);
+        }
+    }
+    // ---- physical derivatives in (lR, lz, phi) ------------------------------
+    double Pv, PR, PZ, Pp, PRR = 0, PZZ = 0, PRZ = 0, PRp = 0, PZp = 0, Ppp = 0;
+    if (P.log_scaling) {
+        const double P0 = -exp(F0);
+        const double P0R = P0 * F0R, P0Z = P0 * F0Z;
+        Pv = P0 * S;
+        PR = fma(P0, SR, P0R * S);
+        PZ = fma(P0, SZ, P0Z * S);
+        Pp = P0 * Sp;
+        if (H) {
+            const double P0RR = P0 * (F0RR + F0R * F0R), P0ZZ = P0 * (F0ZZ + F0Z * F0Z);
+            const double P0RZ = P0 * (F0RZ + F0R * F0Z);
+            PRR = P0 * SRR + 2.0 * P0R * SR + P0RR * S;
+            PZZ = P0 * SZZ + 2.0 * P0Z * SZ + P0ZZ * S;
+            PRZ = P0 * SRZ + P0R * SZ + P0Z * SR + P0RZ * S;
+            PRp = P0 * SRp + P0R * Sp;
+            PZp = P0 * SZp + P0Z * Sp;
+            Ppp = P0 * Spp;
+        }
+    } else {
+        Pv = S; PR = SR; PZ = SZ; Pp = Sp;
+        PRR = SRR; PZZ = SZZ; PRZ = SRZ; PRp = SRp; PZp = SZp; Ppp = Spp;
+    }
+    if (ORD == ORD_POT) { o.phi = Pv; return; }
+    // ---- (lR, lz) -> (R, z): dlX/dX = 1/sqrt(X^2 + Rscale^2) -----------------
+    const double Rs2 = P.Rscale * P.Rscale;
+    const double jR = 1.0 / sqrt(R2 + Rs2), jz = 1.0 / sqrt(fma(z, z, Rs2));
+    const double PhR = PR * jR, PhZ = PZ * jz;
+    double HRR = 0, HZZ = 0, HRZ = 0, HRp = 0, HZp = 0;
+    if (H) {
+        HRR = PRR * jR * jR - PR * R * jR * jR * jR;
+        HZZ = PZZ * jz * jz - PZ * z * jz * jz * jz;
+        HRZ = PRZ * jR * jz;
+        HRp = PRp * jR;
+        HZp = PZp * jz;
+    }
+    if (ORD == ORD_DENS) {
+        // Laplacian in cylindrical coordinates: Phi_RR + Phi_R/R + Phi_pp/R^2 + Phi_zz
+        const double lap = (R > 1.0e-30) ? HRR + PhR / R + Ppp / R2 + HZZ : 2.0 * HRR + HZZ;
+        o.phi = lap * (1.0 / (4.0 * M_PI * G_AGAMA_D));
+        return;
+    }
+    o.phi = Pv;
+    // ---- (R, z, phi) -> Cartesian ---------------------------------------------
+    if (R > 1.0e-30) {
+        const double iR = 1.0 / R;
+        o.g[0] = PhR * cph - Pp * sph * iR;
+        o.g[1] = PhR * sph + Pp * cph * iR;
+    } else {
+        o.g[0] = 0.0; o.g[1] = 0.0;
+    }
+    o.g[2] = PhZ;
+    if (ORD != ORD_HESS) return;
+    if (R > 1.0e-30) {
+        const double iR = 1.0 / R, iR2 = iR * iR;
+        const double c2 = cph * cph, s2 = sph * sph, cs = cph * sph;
+        // d/dx = cos d/dR - sin/R d/dphi, d/dy = sin d/dR + cos/R d/dphi
+        o.h[0] = HRR * c2 - 2.0 * HRp * cs * iR + (PhR * iR) * s2 + (2.0 * Pp * iR2) * cs + Ppp * s2 * iR2;
+        o.h[1] = HRR * s2 + 2.0 * HRp * cs * iR + (PhR * iR) * c2 - (2.0 * Pp * iR2) * cs + Ppp * c2 * iR2;
+        o.h[3] = HRR * cs + HRp * (c2 - s2) * iR - (PhR * iR) * cs - Pp * (c2 - s2) * iR2 - Ppp * cs * iR2;
+        o.h[4] = HRZ * sph + HZp * cph * iR;
+        o.h[5] = HRZ * cph - HZp * sph * iR;
+    } else {
+        o.h[0] = HRR; o.h[1] = HRR; o.h[3] = 0.0; o.h[4] = 0.0; o.h[5] = 0.0;
+    }
+    o.h[2] = HZZ;
+}
+
+inline CSK make_cyl_args(const agx_cylspline *cs)
+{
+    CSK P;
+    P.lR = cs->lR_grid; P.lz = cs->lz_grid; P.nodes = cs->nodes; P.m_arr = cs->m_arr;
+    P.W = cs->W_outer;
+    P.nR = cs->nR; P.nz = cs->nz; P.n_harm = cs->n_harm; P.log_scaling = cs->log_scaling;
+    P.lmax_o = cs->lmax_outer; P.mmax_o = cs->mmax_outer;
+    P.Rscale = cs->Rscale; P.r0 = cs->r0_outer;
+    return P;
+}
+
+}  // namespace agx

@@ -25,7 +25,7 @@
 //  * Table layout is interval-major [k][c][6]: particles in one radial
 //    interval read one contiguous slab (scalar loads in the binned kernel).
 #pragma once
-#include "agx_common.h"
+#include "sph_dev.h"
 
 namespace agx {
 
@@ -38,13 +38,6 @@ struct MPK {
     double inner_s, inner_U, inner_W, outer_s, outer_U, outer_W;
 };
 
-enum { ORD_POT = 0, ORD_GRAD = 1, ORD_HESS = 2, ORD_DENS = 3 };
-
-struct PointOut {
-    double phi;
-    double g[3];   // +grad Phi
-    double h[6];   // +Hessian [xx, yy, zz, xy, yz, xz]
-};
 
 // l=0 extrapolation law and its first two ln r derivatives.
 //   log mode:   Phi = (W + U t) e^{v t}
@@ -242,52 +235,18 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
         return;
     }
     o.phi = Pv;
-    const double Pt = lam * St;
-    const double Pfos = lam * (pole ? Sfp : Sfn * ist);
-    const double ir2 = ir * ir;
-    o.g[0] = fma(Pr * x, ir2, fma(Pt, ct * cph * ir, -Pfos * sph * ir));
-    o.g[1] = fma(Pr * y, ir2, fma(Pt, ct * sph * ir, Pfos * cph * ir));
-    o.g[2] = fma(Pr * z, ir2, -Pt * st * ir);
-    if (ORD != ORD_HESS) return;
-
-    const double Prr = fma(lam * idl * idl, Srr, fma(2.0 * mu * idl, Sr, fma(nu, Sv, d2C0)));
-    const double Ptt = lam * Stt;
-    const double Prt = fma(lam * idl, Srt, mu * St);
-    const double Pf = lam * Sf, Pff = lam * Sff, Ptf = lam * Stf;
-    const double Prf = fma(lam * idl, Srf, mu * Sf);
-    // first derivatives of (ln r, theta, phi); pole-regularised as the reference
-    const double iR = (R > 1.0e-300) ? 1.0 / R : 0.0;
-    const double c3[3] = {x, y, z};
-    const double dl[3] = {x * ir2, y * ir2, z * ir2};
-    const double dt[3] = {ct * cph * ir, ct * sph * ir, -st * ir};
-    const double dp[3] = {-sph * iR, cph * iR, 0.0};
-    const double ir3 = ir2 * ir, ir4 = ir2 * ir2, ir5 = ir4 * ir;
-    const double dct[3] = {-z * x * ir3, -z * y * ir3, ir - z * z * ir3};   // d cos(theta)
-    const double isth = (st > 1.0e-14) ? 1.0 / st : 0.0;
-    const double cos3 = ct * isth * isth * isth;
-    double iR4 = 0.0;
-    if (R > 1.0e-14) iR4 = iR * iR * iR * iR;
-    const int ia[6] = {0, 1, 2, 0, 1, 0}, ib[6] = {0, 1, 2, 1, 2, 2};
-#pragma unroll
-    for (int p = 0; p < 6; ++p) {
-        const int a = ia[p], b = ib[p];
-        const double dab = (a == b) ? 1.0 : 0.0;
-        const double d2lr = dab * ir2 - 2.0 * c3[a] * c3[b] * ir4;
-        const double d2ct = -(dab * z + (a == 2 ? c3[b] : 0.0) + (b == 2 ? c3[a] : 0.0)) * ir3
-                            + 3.0 * z * c3[a] * c3[b] * ir5;
-        const double d2th = -d2ct * isth - dct[a] * dct[b] * cos3;
-        double d2ph = 0.0;
-        if (a == 0 && b == 0) d2ph = 2.0 * x * y * iR4;
-        else if (a == 1 && b == 1) d2ph = -2.0 * x * y * iR4;
-        else if (a == 0 && b == 1) d2ph = (y * y - x * x) * iR4;
-        o.h[p] = Prr * dl[a] * dl[b]
-               + Prt * (dl[a] * dt[b] + dl[b] * dt[a])
-               + Prf * (dl[a] * dp[b] + dl[b] * dp[a])
-               + Ptt * dt[a] * dt[b]
-               + Ptf * (dt[a] * dp[b] + dt[b] * dp[a])
-               + Pff * dp[a] * dp[b]
-               + Pr * d2lr + Pt * d2th + Pf * d2ph;
+    SphDerivs d;
+    d.Pr = Pr;
+    d.Pt = lam * St;
+    d.Pfos = lam * (pole ? Sfp : Sfn * ist);
+    if (ORD == ORD_HESS) {
+        d.Prr = fma(lam * idl * idl, Srr, fma(2.0 * mu * idl, Sr, fma(nu, Sv, d2C0)));
+        d.Ptt = lam * Stt;
+        d.Prt = fma(lam * idl, Srt, mu * St);
+        d.Pf = lam * Sf; d.Pff = lam * Sff; d.Ptf = lam * Stf;
+        d.Prf = fma(lam * idl, Srf, mu * Sf);
     }
+    sph_to_cart<ORD == ORD_HESS>(x, y, z, ir, R, ct, st, cph, sph, d, o.g, o.h);
 }
 
 // Classification of one point: 0 origin, 1 inner law, 2 interior (k), 3 outer law.

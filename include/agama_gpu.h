@@ -83,6 +83,37 @@ int agx_multipole_density(const agx_multipole *mp, const double *d_xyz, int64_t 
                           double inv_4piG, double *d_rho, void *stream);
 
 /* ------------------------------------------------------------------------
+ * CylSpline (azimuthal harmonics x bicubic spline in asinh-scaled R, z).
+ * Tables are built on the host (agama_gpu_amd/cylspline_prep.py, restating
+ * _build_cylspline_data, gpu_potential.py:1143-1253):
+ *   lR_grid [nR], lz_grid [nz]     asinh(R/Rscale), asinh(z/Rscale) (device)
+ *   nodes   [n_harm][nR][nz][4]    f, df/dlR, df/dlz, d2f/dlR dlz (device)
+ *   m_arr   [n_harm]               azimuthal number of each harmonic, sorted
+ *                                  by (|m|, cos before sin) (device)
+ *   W_outer [(lmax_outer+1)^2]     outer PowerLaw coefficients, l(l+1)+m (device)
+ * ---------------------------------------------------------------------- */
+typedef struct agx_cylspline {
+    const double *lR_grid, *lz_grid, *nodes;
+    const int32_t *m_arr;
+    const double *W_outer;
+    int32_t nR, nz, n_harm, log_scaling, lmax_outer, mmax_outer;
+    double Rscale, r0_outer;
+} agx_cylspline;
+
+/* Replaces cylspl_potential_kernel, cylspl_force_kernel, cylspl_hess_kernel
+ * (_cylspl_potential_kernel.cu:871-972) as launched by CylSplinePotentialGPU
+ * (gpu_potential.py:1345-1376). */
+int agx_cylspline_eval(const agx_cylspline *cs, const double *d_xyz, int64_t n,
+                       double *d_phi, double *d_force, double *d_deriv, uint32_t what,
+                       void *stream);
+/* Density = Laplacian / (4 pi G); the reference derives it from the Hessian
+ * trace (gpu_potential.py:1389-1395). */
+int agx_cylspline_density(const agx_cylspline *cs, const double *d_xyz, int64_t n,
+                          double *d_rho, void *stream);
+int64_t agx_cylspline_pack_size(void);
+int agx_cylspline_pack(const agx_cylspline *cs, void *dst, int64_t size);
+
+/* ------------------------------------------------------------------------
  * Fused composite: a sum of components, each evaluated at
  *     x_leaf = scale * (x - center)
  * and accumulated as  Phi += ampl * Phi_leaf,  force += ampl * scale * F_leaf,
@@ -101,7 +132,8 @@ int agx_multipole_density(const agx_multipole *mp, const double *d_xyz, int64_t 
  *   LogHalo        {v0^2, r_c^2, p^2, q^2} DiskAnsatz {4 pi G Sigma, R_d, h, r_in}
  *   UniformAcceleration {a_x, a_y, a_z}
  * Expansion types carry `table`: the device address of a descriptor image
- * written by agx_multipole_pack and copied to device memory.
+ * written by agx_multipole_pack / agx_cylspline_pack and copied to device
+ * memory.
  * ---------------------------------------------------------------------- */
 enum {
     AGX_T_NFW = 1, AGX_T_PLUMMER = 2, AGX_T_HERNQUIST = 3, AGX_T_DEHNEN = 4,

@@ -1,0 +1,105 @@
+"""Pins the CylSpline oracle (oracle/cylspline_oracle.py) without Agama:
+its spline solver against scipy, the product's table builder against it,
+node exactness on the fixture grid, calculus identities inside the grid, and
+harmonicity of the outer PowerLaw expansion (zero Laplacian)."""
+import numpy as np
+import pytest
+from scipy.interpolate import CubicSpline
+
+from conftest import FIXTURES
+from oracle.cylspline_oracle import OracleCylSpline, spline_slopes
+from agama_gpu_amd.coefs import read_coefs
+from agama_gpu_amd.cylspline_prep import build_cylspline_data
+
+FILE = FIXTURES / "600.bar.none_8.coef_cylsp_DR"
+
+
+@pytest.fixture(scope="module")
+def coefs():
+    return read_coefs(FILE)
+
+
+@pytest.fixture(scope="module")
+def orc(coefs):
+    return OracleCylSpline(coefs)
+
+
+def test_spline_slopes_match_scipy():
+    rng = np.random.default_rng(0)
+    x = np.cumsum(rng.uniform(0.1, 1.0, 30))
+    f = np.sin(x) + 0.1 * x ** 2
+    assert np.allclose(spline_slopes(x, f), CubicSpline(x, f, bc_type="natural")(x, 1), rtol=1e-12, atol=1e-12)
+    assert np.allclose(spline_slopes(x, f, 0.0),
+                       CubicSpline(x, f, bc_type=((1, 0.0), "natural"))(x, 1), rtol=1e-12, atol=1e-12)
+
+
+def test_product_table_builder_matches(coefs, orc):
+    d = build_cylspline_data(coefs)
+    assert d["Rscale"] == pytest.approx(orc.Rscale, rel=1e-12)
+    assert np.allclose(d["W_outer"], orc.W, rtol=1e-9, atol=1e-9 * np.abs(orc.W).max())
+    for i, m in enumerate(d["m_arr"]):
+        f, fx, fy, fxy = orc.harm[int(m)]
+        got = d["node_arr"][i]
+        for k, ref in enumerate((f, fx, fy, fxy)):
+            assert np.max(np.abs(got[..., k] - ref)) <= 1e-10 * max(np.abs(ref).max(), 1e-300), (m, k)
+
+
+def test_node_exactness(coefs, orc):
+    R, z = coefs.R_grid, coefs.z_grid
+    rng = np.random.default_rng(1)
+    iR = rng.integers(1, len(R) - 1, 40)
+    iz = rng.integers(1, len(z) - 1, 40)
+    ph = rng.uniform(0, 2 * np.pi, 40)
+    x = np.stack([R[iR] * np.cos(ph), R[iR] * np.sin(ph), z[iz]], axis=1)
+    ex = np.zeros(40)
+    for m, t in coefs.phi.items():
+        T = np.cos(m * ph) if m >= 0 else np.sin(-m * ph)
+        ex += t[iR, iz] * T
+    p = orc.eval(x)[0]
+    assert np.max(np.abs(p - ex) / np.abs(ex)) < 1e-12
+
+
+def _grid_points(orc, n, seed, inside=True):
+    rng = np.random.default_rng(seed)
+    if inside:
+        R = orc.Rscale * np.sinh(rng.uniform(orc.lR[0], orc.lR[-1], n))
+        z = orc.Rscale * np.sinh(rng.uniform(orc.lz[1], orc.lz[-2], n))
+    else:
+        r = np.exp(rng.uniform(np.log(720.0), np.log(3000.0), n))
+        ct = rng.uniform(-1, 1, n)
+        R, z = r * np.sqrt(1 - ct * ct), r * ct
+    ph = rng.uniform(0, 2 * np.pi, n)
+    return np.stack([R * np.cos(ph), R * np.sin(ph), z], axis=1)
+
+
+def test_calculus_inside(orc):
+    x = _grid_points(orc, 300, 2)
+    P, g, H, outer = orc.eval(x)
+    assert not outer.any()
+    step = 1e-6 * np.linalg.norm(x, axis=1)[:, None]
+    e = np.eye(3)
+    fd = np.stack([(orc.eval(x + step * e[k])[0] - orc.eval(x - step * e[k])[0]) / (2 * step[:, 0])
+                   for k in range(3)], axis=1)
+    assert np.max(np.abs(fd - g)) <= 1e-6 * np.abs(g).max()
+    gp = [orc.eval(x + step * e[k])[1] for k in range(3)]
+    gm = [orc.eval(x - step * e[k])[1] for k in range(3)]
+    err = []
+    for s, (i, j) in enumerate([(0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (0, 2)]):
+        fdh = (gp[j][:, i] - gm[j][:, i]) / (2 * step[:, 0])
+        err.append(np.abs(fdh - H[:, s]))
+    err = np.max(np.stack(err, axis=1), axis=1)
+    # a step may cross a spline cell boundary where second derivatives jump
+    assert np.mean(err <= 1e-4 * np.abs(H).max()) > 0.97
+
+
+def test_outer_expansion_is_harmonic(orc):
+    x = _grid_points(orc, 200, 3, inside=False)
+    P, g, H, outer = orc.eval(x)
+    assert outer.all()
+    lap = H[:, 0] + H[:, 1] + H[:, 2]
+    assert np.max(np.abs(lap)) <= 1e-6 * np.abs(H).max()
+    step = 1e-6 * np.linalg.norm(x, axis=1)[:, None]
+    e = np.eye(3)
+    fd = np.stack([(orc.eval(x + step * e[k])[0] - orc.eval(x - step * e[k])[0]) / (2 * step[:, 0])
+                   for k in range(3)], axis=1)
+    assert np.max(np.abs(fd - g)) <= 1e-6 * np.abs(g).max()
