@@ -67,6 +67,19 @@ def classify(desc: dict, x: np.ndarray) -> np.ndarray:
     return kinds
 
 
+def profiled_traffic(workload_file: str):
+    """HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    (copy-calibrated) for this workload, read from its committed summary."""
+    path = os.path.join(ROOT, "profiles", workload_file)
+    try:
+        for ln in open(path):
+            if ln.startswith("traffic_bytes_per_launch"):
+                return int(ln.split("=", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_threads() -> int:
     try:
         n = len(os.sched_getaffinity(0))
@@ -93,7 +106,19 @@ def main():
     ap.add_argument("--n", type=int, default=10_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy three times (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against known traffic)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": src.numel() * 8}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): evaluation of "
               "independent particles shards by contiguous ranges with replicated tables; "
@@ -175,7 +200,9 @@ def main():
                    "table": "triaxial Dehnen gamma=1 M=1e12 a=20 axes 1:0.8:0.6",
                    "points": "Hernquist a=20 seed 0, 1% log-uniform [0.01,2000]"},
         "roofline": {"bound": "valu_fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": profiled_traffic("multipole_halo_lmax8_pot_force_N10M.md"),
+                     "algorithmic_bytes": 56 * N,
                      "kernel_ms": k_mean, "flops_per_launch": flops, "opcount": opinfo},
         "cpu_baseline": cpu,
     }
