@@ -82,6 +82,9 @@ _SIGNATURES = {
                                      C.c_void_p]),
     "agx_composite_density": (C.c_int, [C.POINTER(agx_component), C.c_int, C.c_void_p,
                                         C.c_int64, C.c_void_p, C.c_void_p]),
+    "agx_multipole_adjoint": (C.c_int, [C.POINTER(agx_multipole), C.c_void_p, C.c_int64,
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p]),
     "agx_cylspline_eval": (C.c_int, [C.POINTER(agx_cylspline), C.c_void_p, C.c_int64,
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "agx_cylspline_density": (C.c_int, [C.POINTER(agx_cylspline), C.c_void_p, C.c_int64,

@@ -1,0 +1,246 @@
+"""Differentiable Multipole (+ time-dependent shift) for fitting potentials.
+
+``FittableMultipole`` holds the raw Agama table -- Phi_lm(r_i) and
+dPhi_lm/dr(r_i), the ``2 x n_lm x nR`` numbers of a coefficient file -- as
+torch parameters.  Its forward pass rebuilds the kernel tables from them with
+torch operations (the same steps as ``multipole_prep.build_multipole_data``:
+column order and pruning, inner/outer l=0 laws, log-scaling, the C2 quintic
+solve, Horner coefficients; the discrete choices -- pruning, law modes --
+frozen at construction) and evaluates potential and force with the gfx950
+kernel.  The backward pass is one launch of the adjoint kernel
+(``agx_multipole_adjoint``): a Hessian-vector product for dL/dxyz and a
+wavefront-reduced scatter of dL/d(table) into the spline intervals; torch
+carries the table gradient back through the rebuild to the raw entries.
+
+``SplineCenter`` is the differentiable counterpart of ShiftedPotentialGPU's
+trajectory: the centre at time t is linear in the knot positions, with
+weights from the same cubic spline (not-a-knot, linear extrapolation), so
+gradients reach the knots through x - c(t).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+from .coefs import MultipoleCoefs
+from .composite import _linear_extrapolating_spline
+from .multipole_prep import build_multipole_data
+
+F64 = torch.float64
+
+
+def _quintic_operator(logr: np.ndarray):
+    """A (n x n) and the linear maps B_f, B_d (n x n) with d2 = A^-1 (B_f f + B_d df):
+    the tridiagonal C2-quintic system of multipole_prep.quintic_second_derivs."""
+    n = len(logr)
+    A = np.zeros((n, n))
+    Bf = np.zeros((n, n))
+    Bd = np.zeros((n, n))
+    for i in range(n):
+        if i > 0:
+            h = 1.0 / (logr[i] - logr[i - 1])
+            A[i, i] += 3.0 * h
+            A[i, i - 1] = -h
+            Bf[i, i] -= 20.0 * h ** 3
+            Bf[i, i - 1] += 20.0 * h ** 3
+            Bd[i, i] += 12.0 * h * h
+            Bd[i, i - 1] += 8.0 * h * h
+        if i < n - 1:
+            h = 1.0 / (logr[i + 1] - logr[i])
+            A[i, i] += 3.0 * h
+            A[i, i + 1] = -h
+            Bf[i, i + 1] += 20.0 * h ** 3
+            Bf[i, i] -= 20.0 * h ** 3
+            Bd[i, i] -= 12.0 * h * h
+            Bd[i, i + 1] -= 8.0 * h * h
+    h0 = 1.0 / (logr[1] - logr[0])
+    A[0, :] = 0.0
+    A[0, 0] = 3.0 * h0
+    A[0, 1] = -2.0 * h0
+    Bf[0, :] = 0.0
+    Bd[0, :] = 0.0
+    Bf[0, 1], Bf[0, 0] = 30.0 * h0 ** 3, -30.0 * h0 ** 3
+    Bd[0, 1], Bd[0, 0] = -14.0 * h0 * h0, -16.0 * h0 * h0
+    hn = 1.0 / (logr[n - 1] - logr[n - 2])
+    A[n - 1, :] = 0.0
+    A[n - 1, n - 1] = 3.0 * hn
+    A[n - 1, n - 2] = -2.0 * hn
+    Bf[n - 1, :] = 0.0
+    Bd[n - 1, :] = 0.0
+    Bf[n - 1, n - 1], Bf[n - 1, n - 2] = -30.0 * hn ** 3, 30.0 * hn ** 3
+    Bd[n - 1, n - 2], Bd[n - 1, n - 1] = 14.0 * hn * hn, 16.0 * hn * hn
+    Ainv = np.linalg.inv(A)
+    return Ainv @ Bf, Ainv @ Bd
+
+
+class _MultipoleEval(torch.autograd.Function):
+    """(xyz, poly, scal) -> (phi, force) with the adjoint kernel as backward."""
+
+    @staticmethod
+    def forward(ctx, xyz, poly, scal, owner):
+        desc = owner._desc_for(poly, scal)
+        n = xyz.shape[0]
+        phi = torch.empty(n, dtype=F64, device=xyz.device)
+        force = torch.empty((n, 3), dtype=F64, device=xyz.device)
+        rc = _lib.load().agx_multipole_eval(C.byref(desc), xyz.data_ptr(), n, phi.data_ptr(),
+                                            force.data_ptr(), None, _lib.AGX_PHI | _lib.AGX_FORCE,
+                                            _lib.stream_handle(xyz.device))
+        _lib.check(rc, "agx_multipole_eval")
+        ctx.save_for_backward(xyz, poly, scal)
+        ctx.owner = owner
+        return phi, force
+
+    @staticmethod
+    def backward(ctx, gphi, gforce):
+        xyz, poly, scal = ctx.saved_tensors
+        owner = ctx.owner
+        desc = owner._desc_for(poly, scal)
+        n = xyz.shape[0]
+        gx = torch.empty_like(xyz) if ctx.needs_input_grad[0] else None
+        gpoly = torch.zeros_like(poly)
+        gscal = torch.zeros(5, dtype=F64, device=xyz.device)
+        gphi = None if gphi is None else gphi.contiguous()
+        gforce = None if gforce is None else gforce.contiguous()
+        rc = _lib.load().agx_multipole_adjoint(
+            C.byref(desc), xyz.data_ptr(), n, None if gphi is None else gphi.data_ptr(),
+            None if gforce is None else gforce.data_ptr(), None if gx is None else gx.data_ptr(),
+            gpoly.data_ptr(), gscal.data_ptr(), _lib.stream_handle(xyz.device))
+        _lib.check(rc, "agx_multipole_adjoint")
+        return gx, gpoly, gscal, None
+
+
+class FittableMultipole(torch.nn.Module):
+    """Multipole potential with its raw coefficient table as torch parameters.
+
+    ``phi_table`` and ``dphi_table`` have the file's layout (nR, n_columns),
+    columns ``lm_labels``.  forward(xyz) -> (phi, force), differentiable with
+    respect to xyz and both tables."""
+
+    def __init__(self, coefs: MultipoleCoefs, prune_threshold: float = 1e-16, device=None):
+        super().__init__()
+        dev = torch.device(device) if device is not None else _lib.require_gpu()
+        d = build_multipole_data(coefs, prune_threshold=prune_threshold)
+        if d["resampled"]:
+            raise ValueError("FittableMultipole needs a log-uniform radial grid")
+        self._d = d
+        self.device = dev
+        r = np.asarray(coefs.R_grid, np.float64)
+        self.register_buffer("r", torch.as_tensor(r, device=dev))
+        self.phi_table = torch.nn.Parameter(torch.as_tensor(np.asarray(coefs.phi, np.float64), device=dev))
+        self.dphi_table = torch.nn.Parameter(torch.as_tensor(np.asarray(coefs.dphi_dr, np.float64), device=dev))
+        self.register_buffer("cols", torch.as_tensor(d["columns"], device=dev))
+        Mf, Md = _quintic_operator(np.log(r))
+        self.register_buffer("Mf", torch.as_tensor(Mf, device=dev))
+        self.register_buffer("Md", torch.as_tensor(Md, device=dev))
+        prog = d["program"]
+        self._prog_i = torch.as_tensor(np.concatenate([prog.grp_m, prog.grp_step0, prog.step_cos,
+                                                       prog.step_sin, prog.step_l]).astype(np.int32), device=dev)
+        self._prog_d = torch.as_tensor(np.concatenate([prog.qmm, prog.step_alpha, prog.step_beta]), device=dev)
+        self._n_groups, self._n_steps = len(prog.grp_m), len(prog.step_l)
+
+    # ---- differentiable table build -------------------------------------------
+    def tables(self):
+        """(poly_km (n_int, n_lm, 6), scal [invPhi0, inner U, W, outer U, W])."""
+        d = self._d
+        r = self.r
+        P = self.phi_table[:, self.cols]
+        Dr = self.dphi_table[:, self.cols]
+        dl = Dr * r[:, None]
+        p0, d0 = P[:, 0], Dr[:, 0]
+        zero = torch.zeros((), dtype=F64, device=r.device)
+        log_scaling = bool(d["log_scaling"])
+        invPhi0 = zero
+        U_in, W_in = zero, p0[0]
+        if log_scaling:
+            mode = d["inner_mode"]
+            if mode == "log":
+                U_in = r[0] * d0[0]
+            elif mode == "s2":
+                U_in = 0.5 * r[0] * d0[0]
+                W_in = p0[0] - U_in
+                if not d["inner_reset"]:
+                    invPhi0 = 1.0 / W_in
+        if d["outer_mode"] == "log":
+            U_out, W_out = r[-1] * d0[-1] + p0[-1], p0[-1]
+        else:
+            U_out, W_out = zero, p0[-1]
+        if log_scaling:
+            f0 = torch.log(invPhi0 - 1.0 / p0)
+            df0 = dl[:, 0] / (p0 * (invPhi0 * p0 - 1.0))
+            ratio = P[:, 1:] / p0[:, None]
+            f = torch.cat([f0[:, None], ratio], dim=1)
+            df = torch.cat([df0[:, None], (dl[:, 1:] - ratio * dl[:, 0:1]) / p0[:, None]], dim=1)
+        else:
+            f, df = P, dl
+        d2 = self.Mf @ f + self.Md @ df
+        h = float(d["dlogr"])
+        f0_, f1_ = f[:-1], f[1:]
+        m0, m1 = df[:-1] * h, df[1:] * h
+        q0, q1 = d2[:-1] * h * h, d2[1:] * h * h
+        dlt = f1_ - f0_
+        poly = torch.stack([f0_, m0, 0.5 * q0,
+                            10.0 * dlt - 6.0 * m0 - 4.0 * m1 - 1.5 * q0 + 0.5 * q1,
+                            -15.0 * dlt + 8.0 * m0 + 7.0 * m1 + 1.5 * q0 - q1,
+                            6.0 * dlt - 3.0 * m0 - 3.0 * m1 + 0.5 * (q1 - q0)], dim=-1)
+        scal = torch.stack([invPhi0, U_in, W_in, U_out, W_out])
+        return poly.contiguous(), scal
+
+    def _desc_for(self, poly: torch.Tensor, scal: torch.Tensor):
+        d = self._d
+        sv = scal.detach().cpu().numpy()
+        s = _lib.agx_multipole()
+        s.poly_km = poly.data_ptr()
+        s.prog_i = self._prog_i.data_ptr()
+        s.prog_d = self._prog_d.data_ptr()
+        s.n_int, s.n_lm = d["n_intervals"], d["n_lm"]
+        s.n_groups, s.n_steps, s.lmax = self._n_groups, self._n_steps, d["lmax"]
+        s.log_scaling, s.inner_log, s.outer_log = d["log_scaling"], d["inner_log"], d["outer_log"]
+        s.logr_min, s.dlogr, s.inv_dlogr = d["logr_min"], d["dlogr"], d["inv_dlogr"]
+        s.invPhi0 = float(sv[0])
+        s.inner_s, s.inner_U, s.inner_W = d["inner_s"], float(sv[1]), float(sv[2])
+        s.outer_s, s.outer_U, s.outer_W = d["outer_s"], float(sv[3]), float(sv[4])
+        return s
+
+    def forward(self, xyz: torch.Tensor):
+        poly, scal = self.tables()
+        return _MultipoleEval.apply(xyz.contiguous(), poly, scal, self)
+
+
+class SplineCenter(torch.nn.Module):
+    """Differentiable trajectory centre c(t) from knots (T, 3) at times (T,):
+    the cubic spline of ShiftedPotentialGPU (not-a-knot, linear extrapolation
+    outside [t0, tN]); c(t) = w(t) @ knots with spline weights w(t)."""
+
+    def __init__(self, times, knots, device=None):
+        super().__init__()
+        dev = torch.device(device) if device is not None else _lib.require_gpu()
+        self.times = np.asarray(times, np.float64)
+        self.knots = torch.nn.Parameter(torch.as_tensor(np.asarray(knots, np.float64), device=dev))
+        self._dev = dev
+        self._w_fn = None
+
+    def weights(self, t: float) -> torch.Tensor:
+        if self._w_fn is None:
+            # the spline is linear in its values: interpolating the identity
+            # gives the weight of every knot
+            self._w_fn = _linear_extrapolating_spline(self.times, np.eye(len(self.times)))
+        return torch.as_tensor(np.asarray(self._w_fn(t), np.float64), device=self._dev)
+
+    def forward(self, t: float) -> torch.Tensor:
+        return self.weights(float(t)) @ self.knots
+
+
+class FittableShiftedMultipole(torch.nn.Module):
+    """Multipole at x - c(t): gradients reach the table and the centre knots."""
+
+    def __init__(self, multipole: FittableMultipole, center: SplineCenter):
+        super().__init__()
+        self.multipole = multipole
+        self.center = center
+
+    def forward(self, xyz: torch.Tensor, t: float = 0.0):
+        return self.multipole(xyz - self.center(t)[None, :])

@@ -24,93 +24,27 @@ __global__ __launch_bounds__(kBlock) void mp_lane_kernel(MPK P, const double *__
     }
 }
 
-// ---------------------------------------------------------------------------
-// Radially binned evaluation.  A workgroup takes a tile of particles, sorts
-// the tile's indices by radial interval in LDS (counting sort, each interval
-// padded to whole wavefronts), and then every wavefront evaluates 64 slots
-// that share one interval k.  With k wave-uniform, the interval's coefficient
-// slab is read with scalar loads into SGPRs and feeds the FP64 FMAs directly:
-// no per-lane gathers of the table at all.  Bin 0 collects the particles
-// outside the grid (origin, inner and outer laws), which need no table.
-// ---------------------------------------------------------------------------
-constexpr int kBinThreads = 256;
-constexpr int kMaxBins = 128;          // n_int + 1 bins at most
-constexpr int kMaxTile = 8192;         // particles per tile
-constexpr int kMaxSlots = kMaxTile + kMaxBins * 63;
-constexpr int kMaxChunks = kMaxSlots / 64;
-
 template <int ORD>
 __global__ __launch_bounds__(kBinThreads) void mp_binned_kernel(
     MPK P, const double *__restrict__ xyz, int64_t n, int64_t tile, double *__restrict__ phi,
     double *__restrict__ force, double *__restrict__ deriv, double inv4piG)
 {
-    __shared__ uint16_t slot_idx[kMaxSlots];
-    __shared__ uint8_t pbin[kMaxTile];
-    __shared__ int cnt[kMaxBins], off[kMaxBins + 1], cur[kMaxBins];
-    __shared__ uint8_t chunk_bin[kMaxChunks];
-    __shared__ int n_chunks_s;
-
-    const int nb = P.n_int + 1;
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6, n_waves = kBinThreads / 64;
-    const int64_t n_tiles = (n + tile - 1) / tile;
-
-    for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
-        const int64_t base = t * tile;
-        const int cnt_here = (int)((n - base < tile) ? (n - base) : tile);
-        for (int b = tid; b < nb; b += kBinThreads) { cnt[b] = 0; cur[b] = 0; }
-        __syncthreads();
-        // phase 1: classify and histogram
-        for (int j = tid; j < cnt_here; j += kBinThreads) {
-            const int64_t i = base + j;
-            Where w = classify(P, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
-            const int b = (w.kind == 2) ? w.k + 1 : 0;
-            pbin[j] = (uint8_t)b;
-            atomicAdd(&cnt[b], 1);
+    binned_tiles(P, xyz, n, tile, [&](int b, bool valid, int64_t i) {
+        if (!valid) return;
+        const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+        PointOut o{};
+        if (b > 0) {
+            const int k = b - 1;
+            const double r2 = fma(x, x, fma(y, y, z * z));
+            const double r = sqrt(r2);
+            interior<ORD, true>(P, P.poly + (int64_t)k * P.n_lm * 6, x, y, z, r2, r, log(r), k, o,
+                                inv4piG);
+        } else {
+            Where w = classify(P, x, y, z);
+            extrap<ORD>(P, w, x, y, z, o, inv4piG);
         }
-        __syncthreads();
-        // phase 2: wave-padded offsets, chunk -> bin map
-        if (tid == 0) {
-            int o = 0;
-            for (int b = 0; b < nb; ++b) { off[b] = o; o += (cnt[b] + 63) & ~63; }
-            off[nb] = o;
-            n_chunks_s = o >> 6;
-        }
-        __syncthreads();
-        for (int b = tid; b < nb; b += kBinThreads)
-            for (int c = off[b] >> 6; c < (off[b + 1] >> 6); ++c) chunk_bin[c] = (uint8_t)b;
-        for (int s = tid; s < off[nb]; s += kBinThreads) slot_idx[s] = 0xFFFF;
-        __syncthreads();
-        // phase 3: scatter tile-local indices into their bin's slots
-        for (int j = tid; j < cnt_here; j += kBinThreads) {
-            const int b = pbin[j];
-            const int pos = off[b] + atomicAdd(&cur[b], 1);
-            slot_idx[pos] = (uint16_t)j;
-        }
-        __syncthreads();
-        // phase 4: one interval per wavefront chunk
-        const int nch = n_chunks_s;
-        for (int c = wave; c < nch; c += n_waves) {
-            const int b = __builtin_amdgcn_readfirstlane((int)chunk_bin[c]);
-            const int j = slot_idx[c * 64 + lane];
-            if (j == 0xFFFF) continue;
-            const int64_t i = base + j;
-            const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
-            PointOut o{};
-            if (b > 0) {
-                const int k = b - 1;
-                const double r2 = fma(x, x, fma(y, y, z * z));
-                const double r = sqrt(r2);
-                interior<ORD, true>(P, P.poly + (int64_t)k * P.n_lm * 6, x, y, z, r2, r, log(r), k,
-                                    o, inv4piG);
-            } else {
-                Where w = classify(P, x, y, z);
-                extrap<ORD>(P, w, x, y, z, o, inv4piG);
-            }
-            store<ORD>(o, i, phi, force, deriv);
-        }
-        __syncthreads();
-    }
+        store<ORD>(o, i, phi, force, deriv);
+    });
 }
 
 static int check_mp(const agx_multipole *mp, const double *xyz, int64_t n)

@@ -272,6 +272,7 @@ def build_multipole_data(coefs, prune_threshold: float = 1e-16) -> dict:
     if lmax > LMAX_LIMIT:
         raise ValueError(f"lmax={lmax} exceeds the kernel's compile-time limit of {LMAX_LIMIT}.")
     order = _sort_key_order(lm_l, lm_m)
+    columns = order.copy()
     lm_l, lm_m = lm_l[order], lm_m[order]
     phi, dphi_dlogr = phi[:, order], dphi_dlogr[:, order]
     if not (lm_l[0] == 0 and lm_m[0] == 0):
@@ -282,6 +283,7 @@ def build_multipole_data(coefs, prune_threshold: float = 1e-16) -> dict:
         keep[1:] = np.any(np.abs(phi[:, 1:]) >= prune_threshold, axis=0)
         if not keep.all():
             phi, dphi_dlogr = phi[:, keep], dphi_dlogr[:, keep]
+            columns = columns[keep]
             lm_l, lm_m = lm_l[keep], lm_m[keep]
             n_lm = int(keep.sum())
             lmax = int(lm_l.max())
@@ -329,4 +331,9 @@ def build_multipole_data(coefs, prune_threshold: float = 1e-16) -> dict:
         outer_s=float(outer_s), outer_U=float(outer_U), outer_W=float(outer_W),
         outer_log=int(outer_log),
         program=build_legendre_program(lm_l, lm_m),
+        # structure for a differentiable rebuild of the same tables
+        columns=columns.astype(np.int64), resampled=bool(spread > 1e-3),
+        inner_mode=("log" if inner_log else ("s2" if inner_s == 2.0 else "const")),
+        inner_reset=bool(inner_s == 2.0 and invPhi0 == 0.0),
+        outer_mode=("log" if outer_log else "kepler"),
     )

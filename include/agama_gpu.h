@@ -165,6 +165,20 @@ int agx_composite_density(const agx_component *comps, int n_comps, const double 
 int64_t agx_multipole_pack_size(void);
 int agx_multipole_pack(const agx_multipole *mp, void *dst, int64_t size);
 
+/* Reverse mode of agx_multipole_eval's potential+force outputs.  Given
+ * dL/dPhi (d_gphi, [N], may be NULL) and dL/dF (d_gforce, [N][3], may be
+ * NULL) writes dL/dx into d_gxyz [N][3] (may be NULL; a Hessian-vector
+ * product) and ADDS dL/d(poly_km) into d_gpoly [n_int][n_lm][6] and
+ * dL/d(invPhi0, inner_U, inner_W, outer_U, outer_W) into d_gscal [5]; the
+ * caller zeroes those first.  Table gradients are reduced across each
+ * wavefront before one FP64 atomic add, so their last bits depend on the
+ * order of the atomics.  Needs n_int <= 127.  No reference counterpart (the
+ * reference has no derivatives with respect to positions or tables); it
+ * serves torch.autograd for fitting potentials to tracer forces. */
+int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xyz, int64_t n,
+                          const double *d_gphi, const double *d_gforce, double *d_gxyz,
+                          double *d_gpoly, double *d_gscal, void *stream);
+
 /* Tuning options (process-wide; set before launches, not thread-safe):
  *   AGX_OPT_MP_PATH      Multipole evaluation path: 0 automatic, 1 one particle
  *                        per lane with per-lane table reads, 2 radially binned
