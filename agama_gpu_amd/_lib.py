@@ -10,6 +10,7 @@ import ctypes as C
 import threading
 from pathlib import Path
 
+import numpy as np
 import torch  # noqa: F401  -- loads the HIP runtime the library binds to
 
 PKG = Path(__file__).resolve().parent
@@ -61,6 +62,13 @@ AGX_T_MULTIPOLE, AGX_T_CYLSPLINE = 16, 17
 
 AGX_PHI, AGX_FORCE, AGX_DERIV = 1, 2, 4
 AGX_OPT_MP_PATH, AGX_OPT_MP_TILE, AGX_OPT_MP_BIN_MIN_N, AGX_OPT_MP_BIN_MIN_TERMS = 1, 2, 3, 4
+AGX_OPT_MP_WAVES = 5
+AGX_BATCH_MAX_ITEM = 8192
+AGX_BATCH_LANE, AGX_BATCH_LDS, AGX_BATCH_BINNED = 0, 1, 2
+
+# agx_batch_item as a numpy record (int64 start, int32 count, pot, path, pad)
+BATCH_ITEM_DTYPE = np.dtype([("start", "<i8"), ("count", "<i4"), ("pot", "<i4"),
+                             ("path", "<i4"), ("pad", "<i4")])
 
 
 def set_option(key: int, value: int) -> None:
@@ -93,6 +101,13 @@ _SIGNATURES = {
     "agx_cylspline_pack": (C.c_int, [C.POINTER(agx_cylspline), C.c_void_p, C.c_int64]),
     "agx_multipole_pack_size": (C.c_int64, []),
     "agx_multipole_pack": (C.c_int, [C.POINTER(agx_multipole), C.c_void_p, C.c_int64]),
+    "agx_batch_plan": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
+    "agx_batch_lds_table_doubles": (C.c_int64, []),
+    "agx_multipole_batch_eval": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p,
+                                           C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
+                                           C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
+                                           C.c_void_p]),
 }
 
 

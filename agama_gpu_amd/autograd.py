@@ -26,6 +26,7 @@ import numpy as np
 import torch
 
 from . import _lib
+from .base import norm_device
 from .coefs import MultipoleCoefs
 from .composite import _linear_extrapolating_spline
 from .multipole_prep import build_multipole_data
@@ -118,11 +119,13 @@ class FittableMultipole(torch.nn.Module):
 
     ``phi_table`` and ``dphi_table`` have the file's layout (nR, n_columns),
     columns ``lm_labels``.  forward(xyz) -> (phi, force), differentiable with
-    respect to xyz and both tables."""
+    respect to xyz and both tables.  No column is pruned by default
+    (prune_threshold=0): a harmonic that is zero in the table still has a
+    nonzero gradient."""
 
-    def __init__(self, coefs: MultipoleCoefs, prune_threshold: float = 1e-16, device=None):
+    def __init__(self, coefs: MultipoleCoefs, prune_threshold: float = 0.0, device=None):
         super().__init__()
-        dev = torch.device(device) if device is not None else _lib.require_gpu()
+        dev = norm_device(device) if device is not None else _lib.require_gpu()
         d = build_multipole_data(coefs, prune_threshold=prune_threshold)
         if d["resampled"]:
             raise ValueError("FittableMultipole needs a log-uniform radial grid")
@@ -217,7 +220,7 @@ class SplineCenter(torch.nn.Module):
 
     def __init__(self, times, knots, device=None):
         super().__init__()
-        dev = torch.device(device) if device is not None else _lib.require_gpu()
+        dev = norm_device(device) if device is not None else _lib.require_gpu()
         self.times = np.asarray(times, np.float64)
         self.knots = torch.nn.Parameter(torch.as_tensor(np.asarray(knots, np.float64), device=dev))
         self._dev = dev

@@ -52,7 +52,7 @@ class PotentialBase:
     """Base of all GPU potentials: API methods and ``+`` composition."""
 
     def __init__(self, device=None):
-        self.device = torch.device(device) if device is not None else require_gpu()
+        self.device = norm_device(device) if device is not None else require_gpu()
 
     # ---- primitives -----------------------------------------------------------
     def _leaves(self, t=0.0):
@@ -152,6 +152,15 @@ def alloc_outputs(n: int, device, want_phi: bool, want_force: bool, want_deriv: 
     force = (o[1] if o[1] is not None else torch.empty((n, 3), **kw)) if want_force else None
     deriv = (o[2] if o[2] is not None else torch.empty((n, 6), **kw)) if want_deriv else None
     return phi, force, deriv
+
+
+def norm_device(device) -> torch.device:
+    """torch.device with an explicit index for GPUs ("cuda" -> "cuda:<current>"),
+    so that device comparisons with tensors' devices hold."""
+    d = torch.device(device)
+    if d.type == "cuda" and d.index is None:
+        d = torch.device("cuda", torch.cuda.current_device())
+    return d
 
 
 def ptr(t: Optional[Tensor]):

@@ -48,4 +48,16 @@ inline unsigned grid_for(int64_t n, int per_cu_blocks = 8)
     return (unsigned)want;
 }
 
+// Read a device descriptor image with uniform (scalar) loads.
+template <typename T>
+__device__ __forceinline__ T load_desc(const void *src)
+{
+    static_assert(sizeof(T) % 4 == 0, "descriptor size");
+    T d;
+    int *dst = (int *)&d;
+#pragma unroll
+    for (int q = 0; q < (int)(sizeof(T) / 4); ++q) dst[q] = cldi((const int *)src, q);
+    return d;
+}
+
 }  // namespace agx

@@ -27,18 +27,6 @@ struct CompositeArgs {
     agx_component c[kMaxComponents];
 };
 
-// Read a device descriptor image with uniform (scalar) loads.
-template <typename T>
-__device__ __forceinline__ T load_desc(const void *src)
-{
-    static_assert(sizeof(T) % 4 == 0, "descriptor size");
-    T d;
-    int *dst = (int *)&d;
-#pragma unroll
-    for (int q = 0; q < (int)(sizeof(T) / 4); ++q) dst[q] = cldi((const int *)src, q);
-    return d;
-}
-
 template <int ORD>
 __global__ __launch_bounds__(kBlock) void composite_kernel(CompositeArgs A,
                                                            const double *__restrict__ xyz,

@@ -47,6 +47,36 @@ __global__ __launch_bounds__(kBinThreads) void mp_binned_kernel(
     });
 }
 
+// One particle per lane with the whole coefficient table staged in LDS once
+// per workgroup: the per-term gathers of the lane kernel become LDS reads.
+// Used when the table is small (<= kLdsTableBytes).
+template <int ORD, int WAVES>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES))) void mp_lds_kernel(MPK P, const double *__restrict__ xyz,
+                                                        int64_t n, double *__restrict__ phi,
+                                                        double *__restrict__ force,
+                                                        double *__restrict__ deriv, double inv4piG)
+{
+    extern __shared__ __align__(16) double lds_tab[];
+    const int nv = P.n_int * P.n_lm * 3;               // 16-byte words
+    const dvec2 *src = (const dvec2 *)P.poly;
+    dvec2 *dst = (dvec2 *)lds_tab;
+    for (int j = threadIdx.x; j < nv; j += kBlock) dst[j] = src[j];
+    __syncthreads();
+    const int slab_len = P.n_lm * 6;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+        PointOut o{};
+        Where w = classify(P, x, y, z);
+        if (w.kind == 2)
+            interior<ORD, false>(P, lds_tab + w.k * slab_len, x, y, z, w.r2, w.r, w.lnr, w.k, o,
+                                 inv4piG);
+        else
+            extrap<ORD>(P, w, x, y, z, o, inv4piG);
+        store<ORD>(o, i, phi, force, deriv);
+    }
+}
+
 static int check_mp(const agx_multipole *mp, const double *xyz, int64_t n)
 {
     if (!mp) return AGX_E_NULL;
@@ -59,10 +89,11 @@ static int check_mp(const agx_multipole *mp, const double *xyz, int64_t n)
 }
 
 // Tunables (agx_set_option): evaluation path and binned tile size.
-static int g_mp_path = 0;            // 0 auto, 1 lane kernel, 2 binned kernel
+static int g_mp_path = 0;            // 0 auto, 1 lane, 2 binned, 3 LDS-staged lane
 static int64_t g_mp_tile = 0;        // 0 = automatic
 static int64_t g_mp_bin_min_n = 200000;
 static int64_t g_mp_bin_min_terms = 24;
+static int g_mp_waves = 0;           // LDS path register cap: 0 auto, 1 never, 4 always
 
 struct BinLaunch {
     unsigned grid;
@@ -84,21 +115,46 @@ static BinLaunch bin_launch(int64_t n)
     return BinLaunch{(unsigned)grid, tile};
 }
 
-static bool use_binned(const agx_multipole *mp, int64_t n)
+// Binning pays when the per-term table reads dominate (many active terms) and
+// N is large enough to fill the chip with tiles; with few terms its sort
+// costs more than the reads it removes.
+static bool binned_auto(const agx_multipole *mp, int64_t n)
 {
-    if (mp->n_int + 1 > kMaxBins) return false;
-    if (g_mp_path == 1) return false;
-    if (g_mp_path == 2) return true;
-    // Binning pays when the per-term gathers dominate (many active terms);
-    // with few terms its sort costs more than the gathers it removes.
-    return n >= g_mp_bin_min_n && mp->n_lm >= g_mp_bin_min_terms;
+    return mp->n_int + 1 <= kMaxBins && n >= g_mp_bin_min_n && mp->n_lm >= g_mp_bin_min_terms;
+}
+
+constexpr int64_t kLdsTableBytes = 64 * 1024;
+constexpr int64_t kLdsPerCu = 160 * 1024;
+
+static int64_t table_bytes(const agx_multipole *mp)
+{
+    return (int64_t)mp->n_int * mp->n_lm * 6 * (int64_t)sizeof(double);
 }
 
 template <int ORD>
 static void launch_mp(const agx_multipole *mp, const MPK &P, const double *xyz, int64_t n,
                       double *phi, double *force, double *deriv, double inv4piG, hipStream_t s)
 {
-    if (use_binned(mp, n)) {
+    const int64_t tb = table_bytes(mp);
+    const bool lds_ok = tb <= kLdsTableBytes;
+    int path = g_mp_path;
+    if (path == 0) path = binned_auto(mp, n) ? 2 : (lds_ok ? 3 : 1);
+    if (path == 3 && !lds_ok) path = 1;
+    if (path == 2 && mp->n_int + 1 > kMaxBins) path = 1;
+    if (path == 3) {
+        // potential and force fit 128 VGPRs without spills (4 waves/SIMD);
+        // the Hessian does not (it would spill), so it keeps its registers
+        const bool cap = g_mp_waves == 4 || (g_mp_waves == 0 && (ORD == ORD_POT || ORD == ORD_GRAD));
+        int64_t per_cu = kLdsPerCu / tb;
+        if (per_cu > 8) per_cu = 8;
+        const unsigned grid = grid_for(n, (int)per_cu);
+        if (cap)
+            hipLaunchKernelGGL((mp_lds_kernel<ORD, 4>), dim3(grid), dim3(kBlock), (size_t)tb, s, P,
+                               xyz, n, phi, force, deriv, inv4piG);
+        else
+            hipLaunchKernelGGL((mp_lds_kernel<ORD, 1>), dim3(grid), dim3(kBlock), (size_t)tb, s, P,
+                               xyz, n, phi, force, deriv, inv4piG);
+    } else if (path == 2) {
         BinLaunch bl = bin_launch(n);
         hipLaunchKernelGGL(mp_binned_kernel<ORD>, dim3(bl.grid), dim3(kBinThreads), 0, s, P, xyz,
                            n, bl.tile, phi, force, deriv, inv4piG);
@@ -116,7 +172,7 @@ extern "C" int agx_set_option(int key, int64_t value)
 {
     switch (key) {
     case AGX_OPT_MP_PATH:
-        if (value < 0 || value > 2) return AGX_E_SIZE;
+        if (value < 0 || value > 3) return AGX_E_SIZE;
         g_mp_path = (int)value;
         return AGX_OK;
     case AGX_OPT_MP_TILE:
@@ -130,6 +186,10 @@ extern "C" int agx_set_option(int key, int64_t value)
     case AGX_OPT_MP_BIN_MIN_TERMS:
         if (value < 0) return AGX_E_SIZE;
         g_mp_bin_min_terms = value;
+        return AGX_OK;
+    case AGX_OPT_MP_WAVES:
+        if (value != 0 && value != 1 && value != 4) return AGX_E_SIZE;
+        g_mp_waves = (int)value;
         return AGX_OK;
     default:
         return AGX_E_WHAT;

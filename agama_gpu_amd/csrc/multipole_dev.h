@@ -329,62 +329,75 @@ constexpr int kMaxTile = 8192;         // particles per tile
 constexpr int kMaxSlots = kMaxTile + kMaxBins * 63;
 constexpr int kMaxChunks = kMaxSlots / 64;
 
+// LDS of one binned tile (about 42 KB).
+struct BinLDS {
+    uint16_t slot_idx[kMaxSlots];
+    uint8_t pbin[kMaxTile];
+    int cnt[kMaxBins], off[kMaxBins + 1], cur[kMaxBins];
+    uint8_t chunk_bin[kMaxChunks];
+    int n_chunks;
+};
+
+// One tile: particles [base, base + cnt_here), cnt_here <= kMaxTile.  Called by
+// every thread of the workgroup; ends with a barrier, so L may be reused.
+template <class Body>
+__device__ __forceinline__ void binned_tile(const MPK &P, const double *__restrict__ xyz,
+                                            int64_t base, int cnt_here, BinLDS &L, Body body)
+{
+    const int nb = P.n_int + 1;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6, n_waves = kBinThreads / 64;
+    for (int b = tid; b < nb; b += kBinThreads) { L.cnt[b] = 0; L.cur[b] = 0; }
+    __syncthreads();
+    // phase 1: classify and histogram
+    for (int j = tid; j < cnt_here; j += kBinThreads) {
+        const int64_t i = base + j;
+        Where w = classify(P, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+        const int b = (w.kind == 2) ? w.k + 1 : 0;
+        L.pbin[j] = (uint8_t)b;
+        atomicAdd(&L.cnt[b], 1);
+    }
+    __syncthreads();
+    // phase 2: wave-padded offsets, chunk -> bin map
+    if (tid == 0) {
+        int o = 0;
+        for (int b = 0; b < nb; ++b) { L.off[b] = o; o += (L.cnt[b] + 63) & ~63; }
+        L.off[nb] = o;
+        L.n_chunks = o >> 6;
+    }
+    __syncthreads();
+    for (int b = tid; b < nb; b += kBinThreads)
+        for (int c = L.off[b] >> 6; c < (L.off[b + 1] >> 6); ++c) L.chunk_bin[c] = (uint8_t)b;
+    for (int q = tid; q < L.off[nb]; q += kBinThreads) L.slot_idx[q] = 0xFFFF;
+    __syncthreads();
+    // phase 3: scatter tile-local indices into their bin's slots
+    for (int j = tid; j < cnt_here; j += kBinThreads) {
+        const int b = L.pbin[j];
+        const int pos = L.off[b] + atomicAdd(&L.cur[b], 1);
+        L.slot_idx[pos] = (uint16_t)j;
+    }
+    __syncthreads();
+    // phase 4: one interval per wavefront chunk
+    const int nch = L.n_chunks;
+    for (int c = wave; c < nch; c += n_waves) {
+        const int b = __builtin_amdgcn_readfirstlane((int)L.chunk_bin[c]);
+        const int j = L.slot_idx[c * 64 + lane];
+        body(b, j != 0xFFFF, base + (j == 0xFFFF ? 0 : j));
+    }
+    __syncthreads();
+}
+
+// Tiles of `tile` particles, grid-strided over the workgroups.
 template <class Body>
 __device__ __forceinline__ void binned_tiles(const MPK &P, const double *__restrict__ xyz,
                                              int64_t n, int64_t tile, Body body)
 {
-    __shared__ uint16_t slot_idx[kMaxSlots];
-    __shared__ uint8_t pbin[kMaxTile];
-    __shared__ int cnt[kMaxBins], off[kMaxBins + 1], cur[kMaxBins];
-    __shared__ uint8_t chunk_bin[kMaxChunks];
-    __shared__ int n_chunks_s;
-
-    const int nb = P.n_int + 1;
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6, n_waves = kBinThreads / 64;
+    __shared__ BinLDS L;
     const int64_t n_tiles = (n + tile - 1) / tile;
-
     for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
         const int64_t base = t * tile;
         const int cnt_here = (int)((n - base < tile) ? (n - base) : tile);
-        for (int b = tid; b < nb; b += kBinThreads) { cnt[b] = 0; cur[b] = 0; }
-        __syncthreads();
-        // phase 1: classify and histogram
-        for (int j = tid; j < cnt_here; j += kBinThreads) {
-            const int64_t i = base + j;
-            Where w = classify(P, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
-            const int b = (w.kind == 2) ? w.k + 1 : 0;
-            pbin[j] = (uint8_t)b;
-            atomicAdd(&cnt[b], 1);
-        }
-        __syncthreads();
-        // phase 2: wave-padded offsets, chunk -> bin map
-        if (tid == 0) {
-            int o = 0;
-            for (int b = 0; b < nb; ++b) { off[b] = o; o += (cnt[b] + 63) & ~63; }
-            off[nb] = o;
-            n_chunks_s = o >> 6;
-        }
-        __syncthreads();
-        for (int b = tid; b < nb; b += kBinThreads)
-            for (int c = off[b] >> 6; c < (off[b + 1] >> 6); ++c) chunk_bin[c] = (uint8_t)b;
-        for (int s = tid; s < off[nb]; s += kBinThreads) slot_idx[s] = 0xFFFF;
-        __syncthreads();
-        // phase 3: scatter tile-local indices into their bin's slots
-        for (int j = tid; j < cnt_here; j += kBinThreads) {
-            const int b = pbin[j];
-            const int pos = off[b] + atomicAdd(&cur[b], 1);
-            slot_idx[pos] = (uint16_t)j;
-        }
-        __syncthreads();
-        // phase 4: one interval per wavefront chunk
-        const int nch = n_chunks_s;
-        for (int c = wave; c < nch; c += n_waves) {
-            const int b = __builtin_amdgcn_readfirstlane((int)chunk_bin[c]);
-            const int j = slot_idx[c * 64 + lane];
-            body(b, j != 0xFFFF, base + (j == 0xFFFF ? 0 : j));
-        }
-        __syncthreads();
+        binned_tile(P, xyz, base, cnt_here, L, body);
     }
 }
 

@@ -52,3 +52,60 @@ def halo_lmax8_coefs() -> MultipoleCoefs:
     on [0.1, 500] kpc; harmonics forbidden by triaxial symmetry are exactly 0."""
     rho = dehnen_density(1e12, 20.0, 1.0, 0.8, 0.6)
     return multipole_from_density(rho, 8, 0.1, 500.0, 50, symmetry="triaxial")
+
+
+def ensemble_spec(n_pot: int = 256, seed: int = 3):
+    """Shapes of the ragged ensemble: per potential (lmax in 4..12, radial
+    nodes in 30..80, axis ratios p >= q in [0.6, 1], particle count
+    log-uniform in [1e3, 1e6]).  The draw order is frozen."""
+    rng = np.random.default_rng(seed)
+    lmax = rng.integers(4, 13, n_pot)
+    n_r = rng.integers(30, 81, n_pot)
+    pq = np.sort(rng.uniform(0.6, 1.0, (n_pot, 2)), axis=1)[:, ::-1]
+    counts = np.floor(np.exp(rng.uniform(math.log(1e3), math.log(1e6), n_pot))).astype(np.int64)
+    return [dict(lmax=int(lmax[j]), n_r=int(n_r[j]), p=float(pq[j, 0]), q=float(pq[j, 1]),
+                 count=int(counts[j])) for j in range(n_pot)]
+
+
+def ensemble_coefs(spec: dict) -> MultipoleCoefs:
+    """Triaxial Dehnen (gamma=1, M=1e12, a=20 kpc, axes 1:p:q) on spec's
+    lmax and radial nodes over [0.1, 500] kpc.  The quadrature is coarser
+    than halo_lmax8_coefs' (the ensemble measures throughput; its tables
+    only need to be smooth and of the right shape)."""
+    rho = dehnen_density(1e12, 20.0, 1.0, spec["p"], spec["q"])
+    L = spec["lmax"]
+    return multipole_from_density(rho, L, 0.1, 500.0, spec["n_r"], symmetry="triaxial",
+                                  n_theta=L + 10, n_phi=2 * L + 10, fine_factor=6,
+                                  r_in=1e-3, r_out=5e4)
+
+
+def fit_problem(n: int = 1_000_000, n_knots: int = 200, t_eval: float = 0.73, device=None):
+    """Force-fitting problem: the lmax=8 halo table (all 81 harmonics live)
+    shifted along a cubic-spline centre trajectory of n_knots knots on
+    t in [0, 2] (c(t) = (30 cos wt, 30 sin wt, 3 sin 2 pi t), w = 2 pi / 1.5),
+    evaluated at time t_eval at n Hernquist tracers (seed 2).  The target
+    forces come from a seed-2 perturbed copy of the table (every column
+    scaled by 1 + 0.01 N(0,1)).  Returns (model, xyz, target_force)."""
+    import copy
+
+    import torch
+
+    from .autograd import FittableMultipole, FittableShiftedMultipole, SplineCenter
+    coefs = halo_lmax8_coefs()
+    rng = np.random.default_rng(2)
+    fac = 1.0 + 0.01 * rng.normal(size=coefs.phi.shape[1])
+    target = copy.deepcopy(coefs)
+    target.phi = np.asarray(coefs.phi) * fac[None, :]
+    target.dphi_dr = np.asarray(coefs.dphi_dr) * fac[None, :]
+    times = np.linspace(0.0, 2.0, n_knots)
+    w = 2.0 * math.pi / 1.5
+    knots = np.stack([30.0 * np.cos(w * times), 30.0 * np.sin(w * times),
+                      3.0 * np.sin(2.0 * math.pi * times)], axis=1)
+    xyz = torch.as_tensor(hernquist_points(n, 2), device=device)
+    model = FittableShiftedMultipole(FittableMultipole(coefs, device=device),
+                                     SplineCenter(times, knots, device=device))
+    truth = FittableShiftedMultipole(FittableMultipole(target, device=device),
+                                     SplineCenter(times, knots, device=device))
+    with torch.no_grad():
+        F_target = truth(xyz, t_eval)[1]
+    return model, xyz, F_target

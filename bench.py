@@ -98,6 +98,97 @@ def cpu_model() -> str:
     return "unknown"
 
 
+def bench_batched(dev, reps: int = 5):
+    """Ragged ensemble of 256 Multipole potentials (seed 3: lmax 4..12, 30..80
+    nodes, triaxial Dehnen tables, counts log-uniform in [1e3, 1e6]), pot+force:
+    one persistent launch vs one plain launch per potential, both timed over
+    `reps` passes with inputs resident."""
+    import torch
+    from concurrent.futures import ThreadPoolExecutor
+    from agama_gpu_amd import MultipolePotentialGPU
+    from agama_gpu_amd.batch import MultipoleEnsembleGPU
+    from agama_gpu_amd.workloads import ensemble_coefs, ensemble_spec, hernquist_points
+
+    t_setup = time.perf_counter()
+    spec = ensemble_spec(256, 3)
+    with ThreadPoolExecutor(max_workers=cpu_threads()) as ex:
+        coefs = list(ex.map(ensemble_coefs, spec))
+    pots = [MultipolePotentialGPU(c, device=dev) for c in coefs]
+    counts = [s["count"] for s in spec]
+    n = int(sum(counts))
+    xyz = torch.as_tensor(hernquist_points(n, 3), device=dev)
+    phi = torch.empty(n, dtype=torch.float64, device=dev)
+    force = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    t_tune = time.perf_counter()
+    ens = MultipoleEnsembleGPU(pots, counts, tune=True)
+    t_ready = time.perf_counter()
+    stream = torch.cuda.current_stream(dev)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        w0 = time.perf_counter()
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - w0) * 1e3 / reps, a.elapsed_time(b) / reps
+
+    wall_b, gpu_b = timed(lambda: ens.eval_into(xyz, phi, force))
+    ref = force.clone()
+    wall_s, gpu_s = timed(lambda: ens.eval_separately(xyz, phi, force))
+    agree = float((torch.max(torch.abs(force - ref)) / torch.max(torch.abs(ref))).item())
+    paths = {}
+    for (pth, rows) in ens.choices:
+        k = f"{['lane', 'lds', 'binned'][pth]}/{rows}"
+        paths[k] = paths.get(k, 0) + 1
+    return {"n_potentials": len(pots), "n_particles": n, "work_items": int(len(ens.items)),
+            "ms_batched": wall_b, "ms_batched_events": gpu_b,
+            "ms_per_potential_launches": wall_s, "ms_per_potential_events": gpu_s,
+            "speedup": wall_s / wall_b, "particles_per_s_batched": n / (wall_b * 1e-3),
+            "max_rel_diff_vs_per_potential": agree, "choices": paths,
+            "setup_s": t_tune - t_setup, "tune_s": t_ready - t_tune}
+
+
+def bench_grad(dev, reps: int = 5):
+    """Force fit: Multipole lmax 8 (2 x 81 x 50 live table entries) shifted
+    along a 200-knot spline centre, N=1M tracers; one step = loss (mean
+    squared force residual) forward + backward to the tables, knots and
+    positions."""
+    import torch
+    from agama_gpu_amd.workloads import fit_problem
+    model, xyz, F_t = fit_problem(1_000_000, 200, device=dev)
+    xyz = xyz.clone().requires_grad_(True)
+    params = [model.multipole.phi_table, model.multipole.dphi_table, model.center.knots, xyz]
+
+    def step():
+        for q in params:
+            q.grad = None
+        F = model(xyz, 0.73)[1]
+        loss = ((F - F_t) ** 2).sum(1).mean()
+        loss.backward()
+        return loss
+
+    step()
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    for _ in range(reps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - w0) * 1e3 / reps
+    with torch.no_grad():
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            model(xyz, 0.73)
+        torch.cuda.synchronize()
+        ms_fwd = (time.perf_counter() - t0) * 1e3 / reps
+    return {"n_particles": int(xyz.shape[0]), "table_entries": int(2 * model.multipole.phi_table.numel()),
+            "knots": int(model.center.knots.shape[0]), "ms_forward_backward": ms,
+            "ms_forward": ms_fwd, "particles_per_s_grad": xyz.shape[0] / (ms * 1e-3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +197,8 @@ def main():
     ap.add_argument("--n", type=int, default=10_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched-ensemble and gradient legs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy three times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known traffic)")
@@ -182,6 +275,11 @@ def main():
                "sample": f"full N={N} workload, median of {args.cpu_reps} after 1 warmup; "
                          f"oracle/mp_oracle.c -O3 OpenMP on {cpu_model()}"}
 
+    extras = {}
+    if not args.no_extras:
+        extras["batched"] = bench_batched(dev)
+        extras["grad"] = bench_grad(dev)
+
     line = {
         "metric": "FP64 particles/s (Multipole lmax=8, pot+force, N=10M); batched+grad vs plain",
         "value": value,
@@ -205,6 +303,7 @@ def main():
                      "algorithmic_bytes": 56 * N,
                      "kernel_ms": k_mean, "flops_per_launch": flops, "opcount": opinfo},
         "cpu_baseline": cpu,
+        **extras,
     }
     print(json.dumps(line), flush=True)
 

@@ -179,11 +179,57 @@ int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xyz, int64_t 
                           const double *d_gphi, const double *d_gforce, double *d_gxyz,
                           double *d_gpoly, double *d_gscal, void *stream);
 
+/* ---------------------------------------------------------------------------
+ * Ragged ensembles: many independent Multipole potentials, each with its own
+ * particle range of the concatenated arrays, in one persistent launch.
+ *
+ * agx_batch_plan (host only) cuts potential p's counts[p] rows (ranges are
+ * consecutive, in potential order) into items of at most chunk[p] rows, with
+ * path[p], and orders them by cost[p] * rows, largest first.  With
+ * items == NULL it only returns the item count in *n_items.
+ *
+ * agx_multipole_batch_eval evaluates the items: d_desc holds n_pot descriptor
+ * images (agx_multipole_pack) at desc_stride bytes apart; d_items the plan;
+ * d_xyz [n_rows][3]; outputs as agx_multipole_eval, rows of the concatenation.
+ * d_queue is one int32 of device memory the call resets (by a kernel, not a
+ * memset) and uses as the work counter; concurrent calls need separate
+ * queues.  Items outside [0, n_rows) or naming no potential are skipped; a
+ * path the potential does not fit (LDS table larger than
+ * agx_batch_lds_table_doubles(), n_int > 127 or more than 8192 rows for the
+ * binned path) falls back to AGX_BATCH_LANE.  No reference counterpart (the
+ * reference launches once per potential, gpu_potential.py:757-793).
+ * ------------------------------------------------------------------------- */
+#define AGX_BATCH_MAX_ITEM 8192
+#define AGX_BATCH_LANE 0     /* one particle per lane, table reads from L2 */
+#define AGX_BATCH_LDS 1      /* table staged in LDS per item */
+#define AGX_BATCH_BINNED 2   /* item sorted by radial interval in LDS */
+typedef struct {
+    int64_t start;           /* first row */
+    int32_t count;           /* rows, 1..AGX_BATCH_MAX_ITEM */
+    int32_t pot;             /* potential index */
+    int32_t path;            /* AGX_BATCH_* */
+    int32_t pad;
+} agx_batch_item;
+
+int agx_batch_plan(int32_t n_pot, const int64_t *counts, const int32_t *chunk,
+                   const int32_t *path, const double *cost, agx_batch_item *items,
+                   int64_t capacity, int64_t *n_items);
+int64_t agx_batch_lds_table_doubles(void);
+int agx_multipole_batch_eval(const void *d_desc, int64_t desc_stride, int32_t n_pot,
+                             const agx_batch_item *d_items, int64_t n_items,
+                             const double *d_xyz, int64_t n_rows, double *d_phi,
+                             double *d_force, double *d_deriv, uint32_t what,
+                             int32_t *d_queue, void *stream);
+
 /* Tuning options (process-wide; set before launches, not thread-safe):
- *   AGX_OPT_MP_PATH      Multipole evaluation path: 0 automatic, 1 one particle
+ *   AGX_OPT_MP_PATH      Multipole evaluation path: 0 automatic (binned for
+ *                        many terms and large N, else LDS-staged when the
+ *                        table fits, else per-lane), 1 one particle
  *                        per lane with per-lane table reads, 2 radially binned
  *                        (wave-uniform interval, scalar table reads; needs
- *                        n_int <= 127)
+ *                        n_int <= 127), 3 one particle per lane with the
+ *                        table staged in LDS (tables up to 64 KiB; larger
+ *                        ones take path 1)
  *   AGX_OPT_MP_TILE      particles per binned tile (0 = automatic, <= 8192)
  *   AGX_OPT_MP_BIN_MIN_N smallest N for which the automatic path bins
  *   AGX_OPT_MP_BIN_MIN_TERMS smallest number of active harmonics for which
@@ -194,6 +240,8 @@ int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xyz, int64_t 
 #define AGX_OPT_MP_TILE 2
 #define AGX_OPT_MP_BIN_MIN_N 3
 #define AGX_OPT_MP_BIN_MIN_TERMS 4
+#define AGX_OPT_MP_WAVES 5   /* LDS path register cap for 4 waves/SIMD: 0 automatic
+                                (potential and force), 1 never, 4 always */
 int agx_set_option(int key, int64_t value);
 
 /* Library identification and error text. */

@@ -18,13 +18,14 @@ def _coefs(name):
                                   "lmc": "100.LMC.none_8.coef_mult"}[name])
 
 
+@pytest.mark.parametrize("prune", [0.0, 1e-16])
 @pytest.mark.parametrize("name", ["fire_dense", "lmc", "halo_lmax8"])
-def test_tables_match_numpy_build(name):
+def test_tables_match_numpy_build(name, prune):
     from agama_gpu_amd.autograd import FittableMultipole
     from agama_gpu_amd.multipole_prep import build_multipole_data
     c = _coefs(name)
-    d = build_multipole_data(c)
-    m = FittableMultipole(c, device="cpu")
+    d = build_multipole_data(c, prune_threshold=prune)
+    m = FittableMultipole(c, prune_threshold=prune, device="cpu")
     with torch.no_grad():
         poly, scal = m.tables()
     poly = poly.numpy()
