@@ -26,7 +26,7 @@ class AgxError(RuntimeError):
 
 class agx_multipole(C.Structure):
     _fields_ = [
-        ("poly_km", C.c_void_p), ("prog_i", C.c_void_p), ("prog_d", C.c_void_p),
+        ("poly_km", C.c_void_p), ("program", C.c_void_p),
         ("n_int", C.c_int32), ("n_lm", C.c_int32), ("n_groups", C.c_int32),
         ("n_steps", C.c_int32), ("lmax", C.c_int32),
         ("log_scaling", C.c_int32), ("inner_log", C.c_int32), ("outer_log", C.c_int32),

@@ -29,7 +29,7 @@ from . import _lib
 from .base import norm_device
 from .coefs import MultipoleCoefs
 from .composite import _linear_extrapolating_spline
-from .multipole_prep import build_multipole_data
+from .multipole_prep import build_multipole_data, pack_program
 
 F64 = torch.float64
 
@@ -140,9 +140,7 @@ class FittableMultipole(torch.nn.Module):
         self.register_buffer("Mf", torch.as_tensor(Mf, device=dev))
         self.register_buffer("Md", torch.as_tensor(Md, device=dev))
         prog = d["program"]
-        self._prog_i = torch.as_tensor(np.concatenate([prog.grp_m, prog.grp_step0, prog.step_cos,
-                                                       prog.step_sin, prog.step_l]).astype(np.int32), device=dev)
-        self._prog_d = torch.as_tensor(np.concatenate([prog.qmm, prog.step_alpha, prog.step_beta]), device=dev)
+        self._program = torch.as_tensor(pack_program(prog).view(np.uint8), device=dev)
         self._n_groups, self._n_steps = len(prog.grp_m), len(prog.step_l)
 
     # ---- differentiable table build -------------------------------------------
@@ -197,8 +195,7 @@ class FittableMultipole(torch.nn.Module):
         sv = scal.detach().cpu().numpy()
         s = _lib.agx_multipole()
         s.poly_km = poly.data_ptr()
-        s.prog_i = self._prog_i.data_ptr()
-        s.prog_d = self._prog_d.data_ptr()
+        s.program = self._program.data_ptr()
         s.n_int, s.n_lm = d["n_intervals"], d["n_lm"]
         s.n_groups, s.n_steps, s.lmax = self._n_groups, self._n_steps, d["lmax"]
         s.log_scaling, s.inner_log, s.outer_log = d["log_scaling"], d["inner_log"], d["outer_log"]

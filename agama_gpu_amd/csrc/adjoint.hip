@@ -91,40 +91,43 @@ __device__ __forceinline__ void adjoint_interior(const MPK &P, int k, bool valid
     double Sv = 0.0, Sr = 0.0, St = 0.0, Sx = 0.0;
     double cm = 1.0, sm = 0.0, sp = 1.0, sp1 = 0.0;
     int cur_m = 0;
+    const int S0 = P.n_groups + 1;
     for (int g = 0; g < P.n_groups; ++g) {
-        const int m = cldi(P.grp_m, g);
+        const ivec8 gr = cld_rec(P.prog, g);
+        const int m = gr.s0;
         while (cur_m < m) {
             sp1 = sp; sp *= st;
             const double t = fma(cm, cph, -sm * sph);
             sm = fma(sm, cph, cm * sph); cm = t;
             ++cur_m;
         }
-        const double q = cld(P.qmm, g);
+        const double q = rec_lo(gr);
         double Q1 = q * sp, dQ1 = (m > 0) ? (double)m * q * sp1 * ct : 0.0, Q2 = 0.0, dQ2 = 0.0;
-        const int j0 = cldi(P.grp_step0, g), j1 = cldi(P.grp_step0, g + 1);
+        const int j0 = gr.s1, j1 = gr.s2;
         for (int j = j0; j < j1; ++j) {
-            const int l = cldi(P.step_l, j);
+            const ivec8 st_rec = cld_rec(P.prog, S0 + j);
+            const int l = st_rec.s0;
             if (l > m) {
-                const double al = cld(P.alpha, j), be = cld(P.beta, j);
+                const double al = rec_lo(st_rec), be = rec_hi(st_rec);
                 const double Qn = fma(al * ct, Q1, -be * Q2);
                 const double dQn = fma(al, fma(ct, dQ1, -st * Q1), -be * dQ2);
                 Q2 = Q1; Q1 = Qn; dQ2 = dQ1; dQ1 = dQn;
             }
             for (int side = 0; side < 2; ++side) {
-                const int col = side == 0 ? cldi(P.step_cos, j) : cldi(P.step_sin, j);
-                if (col < 0) continue;
+                const int off = side == 0 ? st_rec.s1 : st_rec.s2;
+                if (off < 0) continue;
                 const double T = side == 0 ? cm : sm;
                 const double dT = side == 0 ? -(double)m * sm : (double)m * cm;
                 const double dTp = side == 0 ? -sm : cm;        // pole limit, m = 1
                 const double X = pole ? ((m == 1) ? dQ1 * dTp : 0.0) : Q1 * dT * ist;
-                cd_t *a = (cd_t *)(slab + col * 6);
+                cd_t *a = (cd_t *)(slab + off);
                 const double C = fma(a[5], s5, fma(a[4], s4, fma(a[3], s3, fma(a[2], s2, fma(a[1], s, a[0])))));
                 const double Cd = fma(5.0 * a[5], s4, fma(4.0 * a[4], s3, fma(3.0 * a[3], s2, fma(2.0 * a[2], s, a[1]))));
                 const double QT = Q1 * T;
                 Sv = fma(C, QT, Sv); Sr = fma(Cd, QT, Sr); St = fma(C, dQ1 * T, St); Sx = fma(C, X, Sx);
                 const double alpha = lam * (wphi * QT + wPt * dQ1 * T + wPf * X) + wPr * mu * QT;
                 const double beta = wPr * lam * idl * QT;
-                scatter6(gslab + col * 6, s, alpha, beta, lane);
+                scatter6(gslab + off, s, alpha, beta, lane);
             }
         }
     }
@@ -217,7 +220,7 @@ extern "C" int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xy
 {
     if (!mp) return AGX_E_NULL;
     if (n < 0) return AGX_E_SIZE;
-    if (!mp->poly_km || !mp->prog_i || !mp->prog_d || !d_gpoly || !d_gscal) return AGX_E_NULL;
+    if (!mp->poly_km || !mp->program || !d_gpoly || !d_gscal) return AGX_E_NULL;
     if (mp->n_int + 1 > kMaxBins) return AGX_E_SIZE;
     if (mp->lmax > 32) return AGX_E_LMAX;
     if (n == 0) return AGX_OK;

@@ -22,6 +22,34 @@ __device__ __forceinline__ int cldi(const int *p, int i)
     return ((cint_t *)p)[i];
 }
 
+// 32-byte program records, read with one scalar load each.
+typedef int ivec8 __attribute__((ext_vector_type(8)));
+typedef int ivec2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(4))) const ivec8 civec8_t;
+
+__device__ __forceinline__ ivec8 cld_rec(const void *base, int idx)
+{
+    return ((civec8_t *)base)[idx];
+}
+// FP64 in dwords 4-5 (lo) or 6-7 (hi) of a record.  The pair is copied into
+// its own vector first: clang's __builtin_bit_cast of a swizzle expression
+// (v.s45) in this toolchain reads the vector's first elements instead of the
+// swizzled ones.
+__device__ __forceinline__ double rec_lo(const ivec8 &v)
+{
+    ivec2 t;
+    t.x = v.s4;
+    t.y = v.s5;
+    return __builtin_bit_cast(double, t);
+}
+__device__ __forceinline__ double rec_hi(const ivec8 &v)
+{
+    ivec2 t;
+    t.x = v.s6;
+    t.y = v.s7;
+    return __builtin_bit_cast(double, t);
+}
+
 constexpr int kBlock = 256;
 
 // Derivative order of an evaluation: potential; +gradient; +Hessian; density.

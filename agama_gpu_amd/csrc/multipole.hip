@@ -82,7 +82,7 @@ static int check_mp(const agx_multipole *mp, const double *xyz, int64_t n)
     if (!mp) return AGX_E_NULL;
     if (n < 0) return AGX_E_SIZE;
     if (n > 0 && !xyz) return AGX_E_NULL;
-    if (!mp->poly_km || !mp->prog_i || !mp->prog_d) return AGX_E_NULL;
+    if (!mp->poly_km || !mp->program) return AGX_E_NULL;
     if (mp->n_int < 1 || mp->n_lm < 1 || mp->n_groups < 0 || mp->n_steps < 0) return AGX_E_SIZE;
     if (mp->lmax > 32) return AGX_E_LMAX;
     return AGX_OK;

@@ -31,8 +31,7 @@ namespace agx {
 
 struct MPK {
     const double *poly;              // [n_int][n_lm][6]
-    const int *grp_m, *grp_step0, *step_cos, *step_sin, *step_l;
-    const double *qmm, *alpha, *beta;
+    const void *prog;                // Legendre program records (agx_multipole.program)
     int n_int, n_lm, n_groups, log_scaling, inner_log, outer_log;
     double logr_min, dlogr, inv_dlogr, invPhi0;
     double inner_s, inner_U, inner_W, outer_s, outer_U, outer_W;
@@ -97,13 +96,13 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
     const double w2 = 2.0 * s, w3 = 3.0 * s2, w4 = 4.0 * s3, w5 = 5.0 * s4;     // C'
     const double u3 = 6.0 * s, u4 = 12.0 * s2, u5 = 20.0 * s3;                  // C''
 
-    auto load6 = [&](int col, double a[6]) {
+    auto load6 = [&](int off, double a[6]) {
         if (UNI) {
-            cdvec2_t *p = (cdvec2_t *)(slab + col * 6);
+            cdvec2_t *p = (cdvec2_t *)(slab + off);
             dvec2 q0 = p[0], q1 = p[1], q2 = p[2];
             a[0] = q0.x; a[1] = q0.y; a[2] = q1.x; a[3] = q1.y; a[4] = q2.x; a[5] = q2.y;
         } else {
-            const dvec2 *p = (const dvec2 *)(slab + col * 6);
+            const dvec2 *p = (const dvec2 *)(slab + off);
             dvec2 q0 = p[0], q1 = p[1], q2 = p[2];
             a[0] = q0.x; a[1] = q0.y; a[2] = q1.x; a[3] = q1.y; a[4] = q2.x; a[5] = q2.y;
         }
@@ -151,8 +150,14 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
 
     double cm = 1.0, sm = 0.0, sp = 1.0, sp1 = 0.0;   // cos/sin(m phi), sin^m, sin^(m-1)
     int cur_m = 0;
+    // Program records are read one iteration ahead (the zero records after
+    // the groups and after the steps keep the look-ahead in bounds), so the
+    // scalar-load latency overlaps the current step's arithmetic.
+    const int S0 = P.n_groups + 1;
+    ivec8 gr = cld_rec(P.prog, 0);
     for (int g = 0; g < P.n_groups; ++g) {
-        const int m = cldi(P.grp_m, g);
+        const ivec8 gn = cld_rec(P.prog, g + 1);
+        const int m = gr.s0;
         while (cur_m < m) {
             sp1 = sp; sp *= st;
             double t = fma(cm, cph, -sm * sph);
@@ -160,16 +165,18 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
             cm = t;
             ++cur_m;
         }
-        const double q = cld(P.qmm, g);
+        const double q = rec_lo(gr);
         double Q1 = q * sp, dQ1 = 0.0, Q2 = 0.0, dQ2 = 0.0;
         if (D1 && m > 0) dQ1 = (double)m * q * sp1 * ct;
         double Vc = 0.0, Vs = 0.0, Rc = 0.0, Rs = 0.0, Tc = 0.0, Ts = 0.0;
         double RRc = 0.0, RRs = 0.0, TTc = 0.0, TTs = 0.0, RTc = 0.0, RTs = 0.0, Lc = 0.0, Ls = 0.0;
-        const int j0 = cldi(P.grp_step0, g), j1 = cldi(P.grp_step0, g + 1);
+        const int j0 = gr.s1, j1 = gr.s2;
+        ivec8 sr = cld_rec(P.prog, S0 + j0);
         for (int j = j0; j < j1; ++j) {
-            const int l = cldi(P.step_l, j);
+            const ivec8 sn = cld_rec(P.prog, S0 + j + 1);
+            const int l = sr.s0;
             if (l > m) {
-                const double al = cld(P.alpha, j), be = cld(P.beta, j);
+                const double al = rec_lo(sr), be = rec_hi(sr);
                 double Qn = fma(al * ct, Q1, -be * Q2);
                 double dQn = 0.0;
                 if (D1) dQn = fma(al, fma(ct, dQ1, -st * Q1), -be * dQ2);
@@ -180,7 +187,7 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
             if (ANG2 && !pole)
                 d2Q = -ct * ist * dQ1 - ((double)(l * (l + 1)) - (double)(m * m) * ist * ist) * Q1;
             const double ll1 = (double)(l * (l + 1));
-            const int cc = cldi(P.step_cos, j), cs = cldi(P.step_sin, j);
+            const int cc = sr.s1, cs = sr.s2;
 #define AGX_TERM(COL, V, Rr, Tt, RRr, TTt, RTt, LL)                           \
             {                                                                 \
                 double a[6];                                                  \
@@ -198,6 +205,7 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
             if (cc >= 0) AGX_TERM(cc, Vc, Rc, Tc, RRc, TTc, RTc, Lc)
             if (cs >= 0) AGX_TERM(cs, Vs, Rs, Ts, RRs, TTs, RTs, Ls)
 #undef AGX_TERM
+            sr = sn;
         }
         // ---- apply cos/sin(m phi) to the group sums ------------------------
         Sv = fma(cm, Vc, fma(sm, Vs, Sv));
@@ -223,6 +231,7 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
             Srt = fma(cm, RTc, fma(sm, RTs, Srt));
         }
         if (ORD == ORD_DENS) SL = fma(cm, Lc, fma(sm, Ls, SL));
+        gr = gn;
     }
 
     // ---- spherical derivatives of the full potential -------------------------
@@ -404,17 +413,9 @@ __device__ __forceinline__ void binned_tiles(const MPK &P, const double *__restr
 inline MPK make_args(const agx_multipole *mp)
 {
     MPK P;
-    const int G = mp->n_groups, S = mp->n_steps;
     P.poly = mp->poly_km;
-    P.grp_m = mp->prog_i;
-    P.grp_step0 = mp->prog_i + G;
-    P.step_cos = mp->prog_i + 2 * G + 1;
-    P.step_sin = P.step_cos + S;
-    P.step_l = P.step_sin + S;
-    P.qmm = mp->prog_d;
-    P.alpha = mp->prog_d + G;
-    P.beta = P.alpha + S;
-    P.n_int = mp->n_int; P.n_lm = mp->n_lm; P.n_groups = G;
+    P.prog = mp->program;
+    P.n_int = mp->n_int; P.n_lm = mp->n_lm; P.n_groups = mp->n_groups;
     P.log_scaling = mp->log_scaling; P.inner_log = mp->inner_log; P.outer_log = mp->outer_log;
     P.logr_min = mp->logr_min; P.dlogr = mp->dlogr; P.inv_dlogr = mp->inv_dlogr;
     P.invPhi0 = mp->invPhi0;

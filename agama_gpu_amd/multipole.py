@@ -20,7 +20,7 @@ import torch
 from . import _lib
 from .base import PotentialBase, alloc_outputs, ptr
 from .coefs import MultipoleCoefs, read_coefs
-from .multipole_prep import INV_4PIG, build_multipole_data
+from .multipole_prep import INV_4PIG, build_multipole_data, pack_program
 
 
 class MultipolePotentialGPU(PotentialBase):
@@ -33,15 +33,10 @@ class MultipolePotentialGPU(PotentialBase):
         prog = d["program"]
         dev = self.device
         self._poly = torch.as_tensor(d["poly_km"].ravel(), device=dev)
-        prog_i = np.concatenate([prog.grp_m, prog.grp_step0, prog.step_cos, prog.step_sin,
-                                 prog.step_l]).astype(np.int32)
-        prog_d = np.concatenate([prog.qmm, prog.step_alpha, prog.step_beta]).astype(np.float64)
-        self._prog_i = torch.as_tensor(prog_i, device=dev)
-        self._prog_d = torch.as_tensor(prog_d, device=dev)
+        self._program = torch.as_tensor(pack_program(prog).view(np.uint8), device=dev)
         s = _lib.agx_multipole()
         s.poly_km = self._poly.data_ptr()
-        s.prog_i = self._prog_i.data_ptr()
-        s.prog_d = self._prog_d.data_ptr()
+        s.program = self._program.data_ptr()
         s.n_int = d["n_intervals"]
         s.n_lm = d["n_lm"]
         s.n_groups = len(prog.grp_m)

@@ -213,6 +213,30 @@ def build_legendre_program(lm_l: np.ndarray, lm_m: np.ndarray) -> LegendreProgra
         np.asarray(ssin, np.int32), np.asarray(sl, np.int32))
 
 
+PROGRAM_RECORD = np.dtype([("i0", "<i4"), ("i1", "<i4"), ("i2", "<i4"), ("i3", "<i4"),
+                           ("d0", "<f8"), ("d1", "<f8")])
+
+
+def pack_program(prog: LegendreProgram) -> np.ndarray:
+    """The kernels' 32-byte program records (layout: agx_multipole.program in
+    include/agama_gpu.h): group records, a zero record, step records (with
+    column offsets 6*c), a zero record."""
+    G, S = len(prog.grp_m), len(prog.step_l)
+    rec = np.zeros(G + S + 2, PROGRAM_RECORD)
+    g = rec[:G]
+    g["i0"] = prog.grp_m
+    g["i1"] = prog.grp_step0[:-1]
+    g["i2"] = prog.grp_step0[1:]
+    g["d0"] = prog.qmm
+    st = rec[G + 1:G + 1 + S]
+    st["i0"] = prog.step_l
+    st["i1"] = np.where(prog.step_cos >= 0, 6 * prog.step_cos, -1)
+    st["i2"] = np.where(prog.step_sin >= 0, 6 * prog.step_sin, -1)
+    st["d0"] = prog.step_alpha
+    st["d1"] = prog.step_beta
+    return rec
+
+
 def build_multipole_data(coefs, prune_threshold: float = 1e-16) -> dict:
     """MultipoleCoefs -> reference-layout tables plus the kernel layout.
 

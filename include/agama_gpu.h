@@ -51,14 +51,24 @@ enum {
  * scalar parameters.
  *   poly_km : [n_int][n_lm][6] Horner coefficients a0..a5 of the scaled
  *             radial functions on interval k, s in [0,1]
- *   prog_i  : grp_m[n_groups], grp_step0[n_groups+1], step_cos[n_steps],
- *             step_sin[n_steps], step_l[n_steps]        (int32, device)
- *   prog_d  : qmm[n_groups], alpha[n_steps], beta[n_steps] (FP64, device)
+ *   program : the Legendre recurrence schedule, 32-byte records (device,
+ *             16-byte aligned), n_groups + n_steps + 2 of them:
+ *       [0, n_groups)   one per active |m| group, in increasing |m|:
+ *                       int32 m, step_begin, step_end, 0;
+ *                       double qmm (mul_m x normalised P_mm / sin^m theta), 0
+ *       n_groups        all zero
+ *       [n_groups + 1, n_groups + 1 + n_steps)   one per recurrence step
+ *                       l = m .. l_last of each group, groups in order:
+ *                       int32 l, cos_off, sin_off, 0; double alpha_l, beta_l
+ *                       (P_l = alpha_l cos(theta) P_{l-1} - beta_l P_{l-2};
+ *                       cos_off / sin_off = 6 x column of the (l, +m) /
+ *                       (l, -m) term in a poly_km slab, -1 if not active)
+ *       last            all zero
+ *             step_begin/step_end index the step records from 0.
  * ---------------------------------------------------------------------- */
 typedef struct agx_multipole {
     const double *poly_km;
-    const int32_t *prog_i;
-    const double *prog_d;
+    const void *program;
     int32_t n_int, n_lm, n_groups, n_steps, lmax;
     int32_t log_scaling, inner_log, outer_log;
     double logr_min, dlogr, inv_dlogr, invPhi0;

@@ -47,7 +47,7 @@ def test_argument_validation_without_gpu():
     d = _lib.agx_multipole()
     assert L.agx_multipole_eval(None, None, 0, None, None, None, 1, None) == -1
     assert L.agx_multipole_eval(C.byref(d), None, 10, None, None, None, 1, None) == -1
-    d.poly_km = d.prog_i = d.prog_d = 8
+    d.poly_km = d.program = 8
     d.n_int, d.n_lm, d.lmax = 4, 1, 40
     assert L.agx_multipole_eval(C.byref(d), 8, 10, 8, None, None, 1, None) == -3
     d.lmax = 0
