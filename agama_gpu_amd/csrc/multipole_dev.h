@@ -25,6 +25,7 @@
 //  * Table layout is interval-major [k][c][6]: particles in one radial
 //    interval read one contiguous slab (scalar loads in the binned kernel).
 #pragma once
+#include <cmath>
 #include "sph_dev.h"
 
 namespace agx {
@@ -35,23 +36,34 @@ struct MPK {
     int n_int, n_lm, n_groups, log_scaling, inner_log, outer_log;
     double logr_min, dlogr, inv_dlogr, invPhi0;
     double inner_s, inner_U, inner_W, outer_s, outer_U, outer_W;
+    double inv_rmin, inv_rmax;       // 1 / r at the first and last node
 };
 
 
 // l=0 extrapolation law and its first two ln r derivatives.
 //   log mode:   Phi = (W + U t) e^{v t}
 //   power mode: Phi = W e^{v t} + U e^{s t}
-__device__ __forceinline__ void law(double v, double s, double U, double W, int logm, double t,
-                                    double &P, double &D, double &D2)
+// t = ln(r / r_edge) and er = r / r_edge = e^t: the exponents that occur in
+// practice (0, -1, 2) are powers of er, which avoids the exponential.
+__device__ __forceinline__ double epow(double a, double t, double er)
 {
-    double ev = (v == 0.0) ? 1.0 : exp(v * t);
+    if (a == 0.0) return 1.0;
+    if (a == -1.0) return rcp_nr(er);
+    if (a == 2.0) return er * er;
+    return exp(a * t);
+}
+
+__device__ __forceinline__ void law(double v, double s, double U, double W, int logm, double t,
+                                    double er, double &P, double &D, double &D2)
+{
+    double ev = epow(v, t, er);
     if (logm) {
         double q = fma(U, t, W);
         P = q * ev;
         D = fma(v, q, U) * ev;
         D2 = fma(v * v, q, 2.0 * v * U) * ev;
     } else {
-        double es = (s != 0.0) ? exp(s * t) : 1.0;
+        double es = epow(s, t, er);
         double wv = W * ev, us = U * es;
         P = wv + us;
         D = fma(v, wv, s * us);
@@ -117,32 +129,16 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
         return fma(a[5], u5, fma(a[4], u4, fma(a[3], u3, 2.0 * a[2])));
     };
 
-    // ---- monopole with the log-scaling inverse ----------------------------
-    double a0[6];
-    load6(0, a0);
-    double C0 = val(a0), dC0 = 0.0, d2C0 = 0.0;
-    if (R1) dC0 = der(a0) * idl;
-    if (D2) d2C0 = der2(a0) * idl * idl;
-    double lam = 1.0, mu = 0.0, nu = 0.0;
-    if (P.log_scaling) {
-        double ex = exp(C0);
-        double P0 = 1.0 / (P.invPhi0 - ex);
-        double dPdX = P0 * P0 * ex;
-        if (D2) d2C0 = dPdX * fma(dC0 * dC0, P0 * (P.invPhi0 + ex), d2C0);
-        if (R1) dC0 = dPdX * dC0;
-        C0 = P0;
-        lam = C0; mu = dC0; nu = d2C0;
-    }
-
     // ---- angular basis ------------------------------------------------------
     const double R2 = fma(x, x, y * y);
     const double R = sqrt(R2);
-    const double ir = 1.0 / r;
+    const double ir = rcp_nr(r);
     const double ct = z * ir, st = R * ir;
     const bool pole = st < 1.0e-10;
-    double cph = 1.0, sph = 0.0;
-    if (R > 1.0e-300) { double iR = 1.0 / R; cph = x * iR; sph = y * iR; }
-    const double ist = pole ? 0.0 : 1.0 / st;
+    double cph = 1.0, sph = 0.0, iR = 0.0;
+    if (R > 1.0e-300) { iR = rcp_nr(R); cph = x * iR; sph = y * iR; }
+    // 1/sin(theta) = r/R, needed inside the term loop only for the Hessian
+    const double ist_loop = (ANG2 && !pole) ? r * iR : 0.0;
 
     // scaled sums over all l>0 terms (trig applied)
     double Sv = 0.0, Sr = 0.0, St = 0.0, Sfn = 0.0, Sfp = 0.0;
@@ -185,7 +181,8 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
             }
             double d2Q = 0.0;
             if (ANG2 && !pole)
-                d2Q = -ct * ist * dQ1 - ((double)(l * (l + 1)) - (double)(m * m) * ist * ist) * Q1;
+                d2Q = -ct * ist_loop * dQ1 -
+                      ((double)(l * (l + 1)) - (double)(m * m) * ist_loop * ist_loop) * Q1;
             const double ll1 = (double)(l * (l + 1));
             const int cc = sr.s1, cs = sr.s2;
 #define AGX_TERM(COL, V, Rr, Tt, RRr, TTt, RTt, LL)                           \
@@ -234,6 +231,24 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
         gr = gn;
     }
 
+    // ---- monopole with the log-scaling inverse (after the term loop, so its
+    // values are not held in registers across it) ----------------------------
+    double a0[6];
+    load6(0, a0);
+    double C0 = val(a0), dC0 = 0.0, d2C0 = 0.0;
+    if (R1) dC0 = der(a0) * idl;
+    if (D2) d2C0 = der2(a0) * idl * idl;
+    double lam = 1.0, mu = 0.0, nu = 0.0;
+    if (P.log_scaling) {
+        double ex = exp(C0);
+        double P0 = rcp_nr(P.invPhi0 - ex);
+        double dPdX = P0 * P0 * ex;
+        if (D2) d2C0 = dPdX * fma(dC0 * dC0, P0 * (P.invPhi0 + ex), d2C0);
+        if (R1) dC0 = dPdX * dC0;
+        C0 = P0;
+        lam = C0; mu = dC0; nu = d2C0;
+    }
+
     // ---- spherical derivatives of the full potential -------------------------
     const double Pv = fma(lam, Sv, C0);
     if (ORD == ORD_POT) { o.phi = Pv; return; }
@@ -244,6 +259,17 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
         return;
     }
     o.phi = Pv;
+    const double ist = pole ? 0.0 : r * iR;
+    if (ORD == ORD_GRAD) {
+        // g = (Pr u_r + Pt u_theta + Pfos u_phi) / r with the unit vectors from
+        // the angles, so x, y, z are not needed after the term loop
+        const double Pt = lam * St, Pf = lam * (pole ? Sfp : Sfn * ist);
+        const double a = fma(Pr, st, Pt * ct);
+        o.g[0] = ir * fma(a, cph, -Pf * sph);
+        o.g[1] = ir * fma(a, sph, Pf * cph);
+        o.g[2] = ir * fma(Pr, ct, -Pt * st);
+        return;
+    }
     SphDerivs d;
     d.Pr = Pr;
     d.Pt = lam * St;
@@ -296,11 +322,12 @@ __device__ __forceinline__ void extrap(const MPK &P, const Where &w, double x, d
         return;
     }
     if (w.kind == 1) {
-        law(0.0, P.inner_s, P.inner_U, P.inner_W, P.inner_log, w.lnr - P.logr_min, Pv, D, D2);
+        law(0.0, P.inner_s, P.inner_U, P.inner_W, P.inner_log, w.lnr - P.logr_min, w.r * P.inv_rmin,
+            Pv, D, D2);
     } else {
         if (ORD == ORD_DENS) { o.phi = 0.0; return; }
         law(-1.0, P.outer_s, P.outer_U, P.outer_W, P.outer_log,
-            w.lnr - fma((double)P.n_int, P.dlogr, P.logr_min), Pv, D, D2);
+            w.lnr - fma((double)P.n_int, P.dlogr, P.logr_min), w.r * P.inv_rmax, Pv, D, D2);
     }
     if (ORD == ORD_DENS) { o.phi = inv4piG * (D2 + D) / w.r2; return; }
     radial_out<ORD>(x, y, z, w.r2, Pv, D, D2, o);
@@ -421,6 +448,8 @@ inline MPK make_args(const agx_multipole *mp)
     P.invPhi0 = mp->invPhi0;
     P.inner_s = mp->inner_s; P.inner_U = mp->inner_U; P.inner_W = mp->inner_W;
     P.outer_s = mp->outer_s; P.outer_U = mp->outer_U; P.outer_W = mp->outer_W;
+    P.inv_rmin = std::exp(-mp->logr_min);
+    P.inv_rmax = std::exp(-(mp->logr_min + (double)mp->n_int * mp->dlogr));
     return P;
 }
 
