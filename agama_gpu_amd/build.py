@@ -26,6 +26,12 @@ HIPCC_FLAGS = [
     "-mcode-object-version=5",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-ffp-contract=fast", "-munsafe-fp-atomics",
+    # Machine-level loop-invariant code motion hoists the FP64 constants of
+    # log/exp and the descriptor fields out of the grid-stride particle loops
+    # into registers held for the whole kernel; in these register-bound FP64
+    # kernels that costs 25-60 VGPRs per kernel (and scratch spills where a
+    # register cap applies).  IR-level LICM still hoists the real work.
+    "-mllvm", "-disable-machine-licm",
     f"-I{INCLUDE}", f"-I{CSRC}",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
