@@ -18,6 +18,9 @@ from .analytic import (AnalyticPotentialGPU, DehnenSphericalPotentialGPU, DiskAn
                        MiyamotoNagaiPotentialGPU, NFWPotentialGPU, PlummerPotentialGPU,
                        UniformAccelerationGPU)
 from .factory import PotentialGPU
+from .batch import MultipoleEnsembleGPU
+from .integrate import LeapfrogGPU
+from .autograd import FittableMultipole, FittableShiftedMultipole, SplineCenter
 
 __all__ = [
     "PotentialGPU", "PotentialBase", "MultipolePotentialGPU", "CylSplinePotentialGPU",
@@ -27,5 +30,7 @@ __all__ = [
     "DehnenSphericalPotentialGPU", "IsochronePotentialGPU", "MiyamotoNagaiPotentialGPU",
     "LogHaloPotentialGPU", "DiskAnsatzPotentialGPU", "UniformAccelerationGPU",
     "MultipoleCoefs", "CylSplineCoefs", "read_coefs", "write_coefs", "G_AGAMA", "INV_4PIG",
+    "MultipoleEnsembleGPU", "LeapfrogGPU", "FittableMultipole", "FittableShiftedMultipole",
+    "SplineCenter",
 ]
 __version__ = "0.1.0"

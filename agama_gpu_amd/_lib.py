@@ -66,6 +66,10 @@ AGX_OPT_MP_WAVES = 5
 AGX_BATCH_MAX_ITEM = 8192
 AGX_BATCH_LANE, AGX_BATCH_LDS, AGX_BATCH_BINNED = 0, 1, 2
 
+# agx_sched_entry as a numpy record (int32 leaf, pad; ampl, scale, center[3])
+SCHED_ENTRY_DTYPE = np.dtype([("leaf", "<i4"), ("pad", "<i4"), ("ampl", "<f8"), ("scale", "<f8"),
+                              ("center", "<f8", (3,))])
+
 # agx_batch_item as a numpy record (int64 start, int32 count, pot, path, pad)
 BATCH_ITEM_DTYPE = np.dtype([("start", "<i8"), ("count", "<i4"), ("pot", "<i4"),
                              ("path", "<i4"), ("pad", "<i4")])
@@ -104,6 +108,10 @@ _SIGNATURES = {
     "agx_batch_plan": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
     "agx_batch_lds_table_doubles": (C.c_int64, []),
+    "agx_batch_max_bins": (C.c_int32, []),
+    "agx_leapfrog": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32,
+                               C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.c_int64, C.c_double, C.c_int32, C.c_void_p]),
     "agx_multipole_batch_eval": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p,
                                            C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
                                            C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,

@@ -6,7 +6,7 @@ LDS and the binned sort -- the tuner times the batched kernel on that potential 
 
   path  AGX_BATCH_LANE    per-lane coefficient reads from L2
         AGX_BATCH_LDS     whole table staged in LDS (only if it fits)
-        AGX_BATCH_BINNED  items sorted by radial interval (n_int <= 127)
+        AGX_BATCH_BINNED  items sorted by radial interval (n_int + 1 <= agx_batch_max_bins())
   rows per item  2048, 4096, 8192
 
 and keeps the fastest.  Results are cached in memory and, when
@@ -51,7 +51,7 @@ def lds_fits(pot) -> bool:
 
 
 def bin_fits(pot) -> bool:
-    return pot.n_intervals + 1 <= 128
+    return pot.n_intervals + 1 <= int(_lib.load().agx_batch_max_bins())
 
 
 def default_choice(pot) -> Tuple[int, int]:

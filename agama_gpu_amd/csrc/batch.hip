@@ -25,10 +25,13 @@
 
 namespace agx {
 
-constexpr int kBatchTableDoubles = (int)(sizeof(BinLDS) / sizeof(double));
+// The tile sort of the batch kernel takes at most 96 bins (n_int <= 95), so
+// that its LDS (about 37 KB) lets four 256-thread workgroups share a CU.
+using BatchBin = BinLDST<kMaxTile, 96>;
+constexpr int kBatchTableDoubles = (int)(sizeof(BatchBin) / sizeof(double));
 
 union BatchLDS {
-    BinLDS bin;
+    BatchBin bin;
     double tab[kBatchTableDoubles];
 };
 
@@ -72,7 +75,7 @@ __global__ __launch_bounds__(kBinThreads) void mp_batch_kernel(
         if (start < 0 || count < 1 || start > n_rows - count || pot < 0 || pot >= n_pot) continue;
         const MPK P = load_desc<MPK>(descs + (int64_t)pot * stride);
         const int tab_len = P.n_int * P.n_lm * 6;
-        if (path == AGX_BATCH_BINNED && P.n_int + 1 <= kMaxBins && count <= kMaxTile) {
+        if (path == AGX_BATCH_BINNED && P.n_int + 1 <= BatchBin::kBins && count <= BatchBin::kTile) {
             binned_tile(P, xyz, start, count, U.bin, [&](int b, bool valid, int64_t i) {
                 if (!valid) return;
                 const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
@@ -135,6 +138,8 @@ static unsigned batch_grid(int64_t n_items)
 using namespace agx;
 
 extern "C" int64_t agx_batch_lds_table_doubles(void) { return kBatchTableDoubles; }
+
+extern "C" int32_t agx_batch_max_bins(void) { return BatchBin::kBins; }
 
 extern "C" int agx_batch_plan(int32_t n_pot, const int64_t *counts, const int32_t *chunk,
                               const int32_t *path, const double *cost, agx_batch_item *items,

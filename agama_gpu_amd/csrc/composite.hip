@@ -14,9 +14,7 @@
 // (A carries Scaled amplitudes, 1/scale factors and Evolving lerp weights),
 // and every particle runs through all leaves once: one read of the positions,
 // one write of each output.
-#include "analytic_dev.h"
-#include "cylspline_dev.h"
-#include "multipole_dev.h"
+#include "leaf_dev.h"
 
 namespace agx {
 
@@ -36,55 +34,13 @@ __global__ __launch_bounds__(kBlock) void composite_kernel(CompositeArgs A,
 {
     constexpr bool G = ORD == ORD_GRAD || ORD == ORD_HESS;
     constexpr bool H = ORD == ORD_HESS;
-    constexpr bool RHO = ORD == ORD_DENS;
     const int64_t stride = (int64_t)gridDim.x * kBlock;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
         const double x0 = xyz[3 * i], y0 = xyz[3 * i + 1], z0 = xyz[3 * i + 2];
         double P = 0.0, g[3] = {0.0, 0.0, 0.0}, h[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         for (int c = 0; c < A.n; ++c) {
             const agx_component &C = A.c[c];
-            const double sg = C.scale;
-            const double x = sg * (x0 - C.center[0]), y = sg * (y0 - C.center[1]),
-                         z = sg * (z0 - C.center[2]);
-            Eval3 e;
-            if (C.type == AGX_T_MULTIPOLE) {
-                const MPK mp = load_desc<MPK>(C.table);
-                PointOut o{};
-                Where w = classify(mp, x, y, z);
-                if (w.kind == 2)
-                    interior<ORD, false>(mp, mp.poly + (int64_t)w.k * mp.n_lm * 6, x, y, z, w.r2,
-                                         w.r, w.lnr, w.k, o, 1.0 / (4.0 * M_PI * G_AGAMA_D));
-                else
-                    extrap<ORD>(mp, w, x, y, z, o, 1.0 / (4.0 * M_PI * G_AGAMA_D));
-                if (RHO) e.rho = o.phi;
-                else e.phi = o.phi;
-                for (int k = 0; k < 3; ++k) e.g[k] = o.g[k];
-                for (int k = 0; k < 6; ++k) e.h[k] = o.h[k];
-            } else if (C.type == AGX_T_CYLSPLINE) {
-                const CSK cs = load_desc<CSK>(C.table);
-                PointOut o{};
-                cylspline_point<ORD>(cs, x, y, z, o);
-                if (RHO) e.rho = o.phi;
-                else e.phi = o.phi;
-                for (int k = 0; k < 3; ++k) e.g[k] = o.g[k];
-                for (int k = 0; k < 6; ++k) e.h[k] = o.h[k];
-            } else {
-                eval_analytic<G, H, RHO>(C.type, C.p, x, y, z, e);
-            }
-            const double a = C.ampl;
-            if (RHO) {
-                P = fma(a * sg * sg, e.rho, P);
-            } else {
-                P = fma(a, e.phi, P);
-                if (G) {
-                    const double ag = a * sg;
-                    for (int k = 0; k < 3; ++k) g[k] = fma(ag, e.g[k], g[k]);
-                }
-                if (H) {
-                    const double ah = a * sg * sg;
-                    for (int k = 0; k < 6; ++k) h[k] = fma(ah, e.h[k], h[k]);
-                }
-            }
+            add_leaf<ORD>(C.type, C.p, C.table, C.ampl, C.scale, C.center, x0, y0, z0, P, g, h);
         }
         if (phi) phi[i] = P;
         if (G && force) { force[3 * i] = -g[0]; force[3 * i + 1] = -g[1]; force[3 * i + 2] = -g[2]; }

@@ -362,23 +362,26 @@ __device__ __forceinline__ void store(const PointOut &o, int64_t i, double *phi,
 constexpr int kBinThreads = 256;
 constexpr int kMaxBins = 128;          // n_int + 1 bins at most
 constexpr int kMaxTile = 8192;         // particles per tile
-constexpr int kMaxSlots = kMaxTile + kMaxBins * 63;
-constexpr int kMaxChunks = kMaxSlots / 64;
 
-// LDS of one binned tile (about 42 KB).
-struct BinLDS {
-    uint16_t slot_idx[kMaxSlots];
-    uint8_t pbin[kMaxTile];
-    int cnt[kMaxBins], off[kMaxBins + 1], cur[kMaxBins];
-    uint8_t chunk_bin[kMaxChunks];
+// LDS of one binned tile: TILE particles at most, BINS bins (n_int + 1) at
+// most.  BinLDS (8192, 128) is about 42 KB.
+template <int TILE, int BINS>
+struct BinLDST {
+    static constexpr int kTile = TILE, kBins = BINS;
+    static constexpr int kSlots = TILE + BINS * 63, kChunks = kSlots / 64;
+    uint16_t slot_idx[kSlots];
+    uint8_t pbin[TILE];
+    int cnt[BINS], off[BINS + 1], cur[BINS];
+    uint8_t chunk_bin[kChunks];
     int n_chunks;
 };
+using BinLDS = BinLDST<kMaxTile, kMaxBins>;
 
 // One tile: particles [base, base + cnt_here), cnt_here <= kMaxTile.  Called by
 // every thread of the workgroup; ends with a barrier, so L may be reused.
-template <class Body>
+template <class LDS, class Body>
 __device__ __forceinline__ void binned_tile(const MPK &P, const double *__restrict__ xyz,
-                                            int64_t base, int cnt_here, BinLDS &L, Body body)
+                                            int64_t base, int cnt_here, LDS &L, Body body)
 {
     const int nb = P.n_int + 1;
     const int tid = threadIdx.x;

@@ -109,3 +109,52 @@ def fit_problem(n: int = 1_000_000, n_knots: int = 200, t_eval: float = 0.73, de
     with torch.no_grad():
         F_target = truth(xyz, t_eval)[1]
     return model, xyz, F_target
+
+
+def evolving_halo_snapshots(n_snap: int = 8):
+    """Growing triaxial halo: Dehnen (gamma=1, a=20 kpc) with mass 0.8 -> 1.0
+    x 1e12 Msol and axes 1 : 0.8 -> 0.9 : 0.6 -> 0.7 over t in [0, 1]; lmax 8,
+    50 nodes on [0.1, 500] kpc (coarse quadrature).  Returns (coefs, times)."""
+    times = np.linspace(0.0, 1.0, n_snap)
+    out = []
+    for i in range(n_snap):
+        f = i / max(1, n_snap - 1)
+        rho = dehnen_density(1e12 * (0.8 + 0.2 * f), 20.0, 1.0, 0.8 + 0.1 * f, 0.6 + 0.1 * f)
+        out.append(multipole_from_density(rho, 8, 0.1, 500.0, 50, symmetry="triaxial",
+                                          n_theta=18, n_phi=26, fine_factor=6,
+                                          r_in=1e-3, r_out=5e4))
+    return out, times
+
+
+def satellite_track(n_knots: int = 50, t_end: float = 1.2):
+    """(T, 4) rows [t, x, y, z]: an infalling satellite, r = 100 - 40 t kpc,
+    angle 2 pi t / 0.8 in a plane inclined by 0.3 rad."""
+    t = np.linspace(0.0, t_end, n_knots)
+    r = 100.0 - 40.0 * t
+    w = 2.0 * math.pi / 0.8
+    c, s = math.cos(0.3), math.sin(0.3)
+    return np.stack([t, r * np.cos(w * t), r * np.sin(w * t) * c, r * np.sin(w * t) * s], axis=1)
+
+
+def satellite_coefs():
+    """Dehnen satellite (gamma=1, M=1e11, a=5 kpc, axes 1:0.9:0.8), lmax 4,
+    30 nodes on [0.05, 200] kpc."""
+    rho = dehnen_density(1e11, 5.0, 1.0, 0.9, 0.8)
+    return multipole_from_density(rho, 4, 0.05, 200.0, 30, symmetry="triaxial",
+                                  n_theta=14, n_phi=18, fine_factor=6, r_in=1e-4, r_out=2e4)
+
+
+def leapfrog_problem(n: int, device=None, n_snap: int = 8, seed: int = 4):
+    """Potential (Evolving halo of n_snap snapshots + Shifted lmax-4 satellite)
+    and initial (x, v): Hernquist positions (a=20 kpc), isotropic Gaussian
+    velocities with 120 km/s per component."""
+    from . import EvolvingPotentialGPU, MultipolePotentialGPU, ShiftedPotentialGPU
+    snaps, times = evolving_halo_snapshots(n_snap)
+    halo = EvolvingPotentialGPU([MultipolePotentialGPU(c, device=device) for c in snaps], times)
+    sat = ShiftedPotentialGPU(MultipolePotentialGPU(satellite_coefs(), device=device),
+                              satellite_track())
+    pot = halo + sat
+    x = hernquist_points(n, seed)
+    rng = np.random.default_rng(seed + 1)
+    v = rng.normal(0.0, 120.0, (n, 3))
+    return pot, x, v

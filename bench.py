@@ -189,6 +189,40 @@ def bench_grad(dev, reps: int = 5):
             "ms_forward": ms_fwd, "particles_per_s_grad": xyz.shape[0] / (ms * 1e-3)}
 
 
+def bench_leapfrog(dev, n: int = 1_000_000, n_steps: int = 1000, eager_steps: int = 50):
+    """Leapfrog of n test particles for n_steps through an Evolving halo of 8
+    lmax-8 snapshots plus a spline-shifted lmax-4 satellite: HIP-graph replay
+    (50 steps per graph, device-resident step counter) vs the same steps
+    through the public API one at a time (time from the host), the latter on
+    the first `eager_steps` steps."""
+    import torch
+    from agama_gpu_amd import LeapfrogGPU
+    from agama_gpu_amd.workloads import leapfrog_problem
+    pot, x, v = leapfrog_problem(n, device=dev)
+    lf = LeapfrogGPU(pot, dt=1e-3, n_steps=n_steps)
+    xt = torch.as_tensor(x, device=dev)
+    vt = torch.as_tensor(v, device=dev)
+    lf.run(xt, vt, n_steps=50, graph=True, steps_per_graph=50)     # capture + warm-up
+    torch.cuda.synchronize()
+    lf.set_state(xt, vt)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    lf.advance(n_steps, graph=True, steps_per_graph=50)
+    torch.cuda.synchronize()
+    ms_graph = (time.perf_counter() - w0) * 1e3 / n_steps
+    xg = lf.x.clone()
+    lf.run_eager(xt, vt, 2)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    xe, _ = lf.run_eager(xt, vt, eager_steps)
+    torch.cuda.synchronize()
+    ms_eager = (time.perf_counter() - w0) * 1e3 / eager_steps
+    return {"n_particles": n, "n_steps": n_steps, "leaves_per_step": lf.max_active,
+            "ms_per_step_graph": ms_graph, "ms_per_step_eager": ms_eager,
+            "speedup": ms_eager / ms_graph, "particle_steps_per_s": n / (ms_graph * 1e-3),
+            "final_state_finite": bool(torch.isfinite(xg).all().item())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -279,6 +313,7 @@ def main():
     if not args.no_extras:
         extras["batched"] = bench_batched(dev)
         extras["grad"] = bench_grad(dev)
+        extras["leapfrog"] = bench_leapfrog(dev)
 
     line = {
         "metric": "FP64 particles/s (Multipole lmax=8, pot+force, N=10M); batched+grad vs plain",

@@ -205,8 +205,8 @@ int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xyz, int64_t 
  * memset) and uses as the work counter; concurrent calls need separate
  * queues.  Items outside [0, n_rows) or naming no potential are skipped; a
  * path the potential does not fit (LDS table larger than
- * agx_batch_lds_table_doubles(), n_int > 127 or more than 8192 rows for the
- * binned path) falls back to AGX_BATCH_LANE.  No reference counterpart (the
+ * agx_batch_lds_table_doubles(), n_int + 1 > agx_batch_max_bins() or more
+ * than 8192 rows for the binned path) falls back to AGX_BATCH_LANE.  No reference counterpart (the
  * reference launches once per potential, gpu_potential.py:757-793).
  * ------------------------------------------------------------------------- */
 #define AGX_BATCH_MAX_ITEM 8192
@@ -225,11 +225,40 @@ int agx_batch_plan(int32_t n_pot, const int64_t *counts, const int32_t *chunk,
                    const int32_t *path, const double *cost, agx_batch_item *items,
                    int64_t capacity, int64_t *n_items);
 int64_t agx_batch_lds_table_doubles(void);
+int32_t agx_batch_max_bins(void);    /* AGX_BATCH_BINNED needs n_int + 1 <= this */
 int agx_multipole_batch_eval(const void *d_desc, int64_t desc_stride, int32_t n_pot,
                              const agx_batch_item *d_items, int64_t n_items,
                              const double *d_xyz, int64_t n_rows, double *d_phi,
                              double *d_force, double *d_deriv, uint32_t what,
                              int32_t *d_queue, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Leapfrog (kick-drift-kick) of N test particles through a time-dependent
+ * potential, one launch per step, for capture in a HIP graph.
+ *
+ * The potential at step s (time t0 + (s+1) dt) is a schedule row: entries
+ * d_sched[s * max_active + e], e < min(d_count[s], max_active), each naming a
+ * leaf of d_leaves (type, parameters and table of an agx_component; its own
+ * ampl/scale/center fields are ignored) and giving that leaf's weight, scale
+ * and centre at that time (the flattening described for agx_component).
+ * d_step is a device int32 step counter: every step reads it and then
+ * advances it by one (by a kernel), so replaying a captured graph of
+ * agx_leapfrog launches continues the integration.  Steps with a counter
+ * outside [0, n_sched) leave the particles untouched.
+ * d_x, d_v, d_a [N][3]: positions, velocities and the acceleration at the
+ * current positions and time (the caller evaluates it before the first
+ * step); all three are updated in place.  No reference counterpart (the
+ * reference offers evaluation only; integrators are its callers'). */
+typedef struct {
+    int32_t leaf, pad;
+    double ampl, scale, center[3];
+} agx_sched_entry;
+
+int agx_leapfrog(const agx_component *d_leaves, int32_t n_leaves,
+                 const agx_sched_entry *d_sched, const int32_t *d_count,
+                 int32_t max_active, int64_t n_sched, int32_t *d_step,
+                 double *d_x, double *d_v, double *d_a, int64_t n, double dt,
+                 int32_t n_steps, void *stream);
 
 /* Tuning options (process-wide; set before launches, not thread-safe):
  *   AGX_OPT_MP_PATH      Multipole evaluation path: 0 automatic (binned for
