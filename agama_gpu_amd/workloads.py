@@ -158,3 +158,86 @@ def leapfrog_problem(n: int, device=None, n_snap: int = 8, seed: int = 4):
     rng = np.random.default_rng(seed + 1)
     v = rng.normal(0.0, 120.0, (n, 3))
     return pot, x, v
+
+
+def high_order_coefs(lmax: int = 32, n_r: int = 80, seed: int = 1) -> MultipoleCoefs:
+    """High-order Multipole: a Hernquist density (M=1e12, a=20 kpc) with a
+    seeded random triaxial stretch (axis ratios in [0.7, 1]) and a random
+    angular perturbation sum_{l>=1,m} eps_lm 0.3^l Y_lm (l=1 included:
+    lopsided), eps ~ 0.3 N(0,1); expanded to lmax on n_r log nodes over
+    [0.1, 500] kpc (no symmetry: all (lmax+1)^2 harmonics)."""
+    from .expansion import real_basis
+    rng = np.random.default_rng(seed)
+    p, q = np.sort(rng.uniform(0.7, 1.0, 2))[::-1]
+    labels = [(l, m) for l in range(lmax + 1) for m in range(-l, l + 1)]
+    eps = np.array([0.0 if l == 0 else 0.3 * rng.normal() * 0.3 ** l for l, m in labels])
+    M, a = 1e12, 20.0
+
+    cache = {}
+
+    def angular(x, y, z):
+        # the perturbation depends on the direction only; the quadrature
+        # calls rho on (radii x directions) blocks sharing their directions
+        key = x.shape[-1]
+        if key not in cache:
+            x0, y0, z0 = x.reshape(-1, key)[0], y.reshape(-1, key)[0], z.reshape(-1, key)[0]
+            r0 = np.sqrt(x0 * x0 + y0 * y0 + z0 * z0)
+            B = real_basis(lmax, z0 / r0, np.arctan2(y0, x0))
+            cache[key] = 1.0 + eps @ B
+        return cache[key]
+
+    def rho(x, y, z):
+        mm = np.sqrt(x * x + (y / p) ** 2 + (z / q) ** 2)
+        base = M * a / (2.0 * math.pi * p * q) / (np.maximum(mm, 1e-300) * (mm + a) ** 3)
+        return base * angular(x, y, z)
+
+    return multipole_from_density(rho, lmax, 0.1, 500.0, n_r, symmetry="none",
+                                  fine_factor=8, r_in=1e-3, r_out=5e4)
+
+
+def mw_disk_bar_coefs(nR: int = 32, nz: int = 32, n_phi: int = 16):
+    """CylSpline table of a Miyamoto-Nagai disk (M=6e10, a=3, b=0.3 kpc) plus
+    a quadrupole bar (Dehnen 2000 form: R_b = 3.5 kpc, strength 0.01 of
+    (220 km/s)^2 at R_0 = 8 kpc, angle 0.3 rad) with weaker m=4 and m=6
+    companions, sampled on a 32 x 32 grid uniform in asinh(R/3), asinh(z/0.3)
+    up to R = 200, z = 200 kpc (z >= 0 half; mirrored by the builder), and
+    split into azimuthal harmonics m = 0..6 by an exact 16-point FFT."""
+    from .coefs import CylSplineCoefs
+    from .multipole_prep import G_AGAMA
+    R = 3.0 * np.sinh(np.linspace(0.0, np.arcsinh(200.0 / 3.0), nR))
+    z = 0.3 * np.sinh(np.linspace(0.0, np.arcsinh(200.0 / 0.3), nz))
+    ph = np.arange(n_phi) * (2.0 * math.pi / n_phi)
+    RR, ZZ, PP = np.meshgrid(R, z, ph, indexing="ij")
+    Mg = G_AGAMA * 6e10
+    phi_mn = -Mg / np.sqrt(RR * RR + (3.0 + np.sqrt(ZZ * ZZ + 0.09)) ** 2)
+    r = np.sqrt(RR * RR + ZZ * ZZ)
+    Rb, amp = 3.5, 0.01 * 220.0 ** 2 / 3.0 * (8.0 / 3.5) ** 3
+    u = np.where(r >= Rb, -(np.maximum(r, Rb) / Rb) ** -3, (r / Rb) ** 3 - 2.0)
+    ang = np.where(r > 0, (RR / np.maximum(r, 1e-300)) ** 2, 0.0)
+    shape = amp * u * ang
+    phi_bar = shape * (np.cos(2 * (PP - 0.3)) + 0.15 * np.cos(4 * (PP - 0.3)) * ang +
+                       0.05 * np.cos(6 * (PP - 0.5)) * ang * ang)
+    total = phi_mn + phi_bar
+    F = np.fft.rfft(total, axis=2) / n_phi
+    phi = {0: F[:, :, 0].real.copy()}
+    floor = 1e-12 * np.max(np.abs(phi[0]))      # FFT round-off of absent harmonics
+    for m in range(1, 7):
+        c = 2.0 * F[:, :, m].real
+        s = -2.0 * F[:, :, m].imag
+        if np.max(np.abs(c)) > floor:
+            phi[m] = c.copy()
+        if np.max(np.abs(s)) > floor:
+            phi[-m] = s.copy()
+    return CylSplineCoefs(R, z, phi, 6)
+
+
+def mw_composite(device=None):
+    """Milky-Way-like composite: Multipole halo (halo_lmax8_coefs), CylSpline
+    disk + bar (mw_disk_bar_coefs), NFW (M=1e12, rs=20 kpc), Plummer bulge
+    (M=1e10, b=0.5 kpc)."""
+    from . import (CylSplinePotentialGPU, MultipolePotentialGPU, NFWPotentialGPU,
+                   PlummerPotentialGPU)
+    return (MultipolePotentialGPU(halo_lmax8_coefs(), device=device) +
+            CylSplinePotentialGPU(mw_disk_bar_coefs(), device=device) +
+            NFWPotentialGPU(mass=1e12, scaleRadius=20.0, device=device) +
+            PlummerPotentialGPU(mass=1e10, scaleRadius=0.5, device=device))
