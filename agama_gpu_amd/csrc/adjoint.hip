@@ -47,7 +47,7 @@ __device__ __forceinline__ void adjoint_interior(const MPK &P, int k, bool valid
         wphi = 0.0;
     }
     const double wfx = valid ? wf[0] : 0.0, wfy = valid ? wf[1] : 0.0, wfz = valid ? wf[2] : 0.0;
-    const double r2 = fma(x, x, fma(y, y, z * z)), r = sqrt(r2), lnr = log(r);
+    const double r2 = fma(x, x, fma(y, y, z * z)), r = sqrt(r2), lnr = agx_log(r);
     // Hessian-vector product for dL/dx
     {
         PointOut o{};

@@ -61,6 +61,35 @@ __device__ __forceinline__ double rcp_nr(double x)
     return r;
 }
 
+// Natural logarithm of a positive normal x, within a few ulp: x = m 2^e with
+// m in [sqrt(1/2), sqrt(2)), ln m = 2 atanh(f), f = (m-1)/(m+1), |f| <= 0.1716,
+// by its odd series to f^21 (the next term is below 2^-53 of the sum).  About
+// 30 instructions against about 90 for the library log, whose extra work
+// handles zero, subnormal, negative and non-finite inputs that do not occur
+// where this is used.
+__device__ __forceinline__ double agx_log(double x)
+{
+    double m = __builtin_amdgcn_frexp_mant(x);     // [0.5, 1)
+    int e = __builtin_amdgcn_frexp_exp(x);
+    if (m < 0.70710678118654752440) { m += m; --e; }
+    const double f = (m - 1.0) * rcp_nr(m + 1.0);
+    const double f2 = f * f;
+    double p = 1.0 / 21.0;
+    p = fma(p, f2, 1.0 / 19.0);
+    p = fma(p, f2, 1.0 / 17.0);
+    p = fma(p, f2, 1.0 / 15.0);
+    p = fma(p, f2, 1.0 / 13.0);
+    p = fma(p, f2, 1.0 / 11.0);
+    p = fma(p, f2, 1.0 / 9.0);
+    p = fma(p, f2, 1.0 / 7.0);
+    p = fma(p, f2, 1.0 / 5.0);
+    p = fma(p, f2, 1.0 / 3.0);
+    const double lnm = fma(2.0 * f * f2, p, 2.0 * f);
+    const double de = (double)e;
+    // ln 2 split so that e * LN2_HI is exact for |e| < 2^11
+    return fma(de, 6.93147180369123816490e-01, fma(de, 1.90821492927058770002e-10, lnm));
+}
+
 constexpr int kBlock = 256;
 
 // Derivative order of an evaluation: potential; +gradient; +Hessian; density.

@@ -84,7 +84,7 @@ __global__ __launch_bounds__(kBinThreads) void mp_batch_kernel(
                     const int k = b - 1;
                     const double r2 = fma(x, x, fma(y, y, z * z));
                     const double r = sqrt(r2);
-                    interior<ORD, true>(P, P.poly + (int64_t)k * P.n_lm * 6, x, y, z, r2, r, log(r),
+                    interior<ORD, true>(P, P.poly + (int64_t)k * P.n_lm * 6, x, y, z, r2, r, agx_log(r),
                                         k, o, 0.0);
                 } else {
                     Where w = classify(P, x, y, z);

@@ -154,6 +154,15 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
     for (int g = 0; g < P.n_groups; ++g) {
         const ivec8 gn = cld_rec(P.prog, g + 1);
         const int m = gr.s0;
+        if (m - cur_m == 2) {
+            // two steps at once (tables with even-m symmetry): rotate by 2 phi
+            const double c2 = fma(cph, cph, -sph * sph), s2 = 2.0 * cph * sph;
+            sp1 = sp * st; sp = sp1 * st;
+            const double t = fma(cm, c2, -sm * s2);
+            sm = fma(sm, c2, cm * s2);
+            cm = t;
+            cur_m = m;
+        }
         while (cur_m < m) {
             sp1 = sp; sp *= st;
             double t = fma(cm, cph, -sm * sph);
@@ -162,47 +171,84 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
             ++cur_m;
         }
         const double q = rec_lo(gr);
-        double Q1 = q * sp, dQ1 = 0.0, Q2 = 0.0, dQ2 = 0.0;
-        if (D1 && m > 0) dQ1 = (double)m * q * sp1 * ct;
+        // Two recurrence buffers A and B are used alternately: each step
+        // writes the new P_l over the older of the two, so no register copies
+        // are needed.  With scalar (UNI) coefficient loads the cos-term
+        // coefficients of the next step are loaded into the other buffer (in
+        // SGPRs) while this step computes; per-lane loads are issued at their
+        // use, where prefetching costs more VGPRs than it saves in latency.
+        double QA = q * sp, dQA = 0.0, QB = 0.0, dQB = 0.0;
+        if (D1 && m > 0) dQA = (double)m * q * sp1 * ct;
         double Vc = 0.0, Vs = 0.0, Rc = 0.0, Rs = 0.0, Tc = 0.0, Ts = 0.0;
         double RRc = 0.0, RRs = 0.0, TTc = 0.0, TTs = 0.0, RTc = 0.0, RTs = 0.0, Lc = 0.0, Ls = 0.0;
-        const int j0 = gr.s1, j1 = gr.s2;
-        ivec8 sr = cld_rec(P.prog, S0 + j0);
-        for (int j = j0; j < j1; ++j) {
-            const ivec8 sn = cld_rec(P.prog, S0 + j + 1);
-            const int l = sr.s0;
-            if (l > m) {
-                const double al = rec_lo(sr), be = rec_hi(sr);
-                double Qn = fma(al * ct, Q1, -be * Q2);
-                double dQn = 0.0;
-                if (D1) dQn = fma(al, fma(ct, dQ1, -st * Q1), -be * dQ2);
-                Q2 = Q1; Q1 = Qn;
-                dQ2 = dQ1; dQ1 = dQn;
+        auto add_term = [&](const double a[6], double Q, double dQ, double d2Q, double ll1, double &V,
+                            double &Rr, double &Tt, double &RRr, double &TTt, double &RTt, double &LL) {
+            const double C = val(a);
+            V = fma(C, Q, V);
+            if (R1) {
+                const double dC = der(a);
+                Rr = fma(dC, Q, Rr);
+                if (D1) Tt = fma(C, dQ, Tt);
+                if (ANG2) RTt = fma(dC, dQ, RTt);
             }
+            if (D2) RRr = fma(der2(a), Q, RRr);
+            if (ANG2) TTt = fma(C, d2Q, TTt);
+            if (ORD == ORD_DENS) LL = fma(ll1 * C, Q, LL);
+        };
+        // the cos and sin terms of one step (degree rec.s0, current Q, dQ);
+        // the cos coefficients were loaded into ca beforehand
+        auto terms = [&](const ivec8 &rec, double Q, double dQ, const double ca[6]) {
+            const int l = rec.s0;
             double d2Q = 0.0;
             if (ANG2 && !pole)
-                d2Q = -ct * ist_loop * dQ1 -
-                      ((double)(l * (l + 1)) - (double)(m * m) * ist_loop * ist_loop) * Q1;
+                d2Q = -ct * ist_loop * dQ -
+                      ((double)(l * (l + 1)) - (double)(m * m) * ist_loop * ist_loop) * Q;
             const double ll1 = (double)(l * (l + 1));
-            const int cc = sr.s1, cs = sr.s2;
-#define AGX_TERM(COL, V, Rr, Tt, RRr, TTt, RTt, LL)                           \
-            {                                                                 \
-                double a[6];                                                  \
-                load6(COL, a);                                                \
-                const double C = val(a);                                      \
-                V = fma(C, Q1, V);                                            \
-                if (R1) { const double dC = der(a);                           \
-                    Rr = fma(dC, Q1, Rr);                                     \
-                    if (D1) Tt = fma(C, dQ1, Tt);                             \
-                    if (ANG2) RTt = fma(dC, dQ1, RTt); }                      \
-                if (D2) { RRr = fma(der2(a), Q1, RRr); }                      \
-                if (ANG2) TTt = fma(C, d2Q, TTt);                             \
-                if (ORD == ORD_DENS) LL = fma(ll1 * C, Q1, LL);               \
+            if (rec.s1 >= 0) {
+                if (UNI) {
+                    add_term(ca, Q, dQ, d2Q, ll1, Vc, Rc, Tc, RRc, TTc, RTc, Lc);
+                } else {
+                    double ac[6];
+                    load6(rec.s1, ac);
+                    add_term(ac, Q, dQ, d2Q, ll1, Vc, Rc, Tc, RRc, TTc, RTc, Lc);
+                }
             }
-            if (cc >= 0) AGX_TERM(cc, Vc, Rc, Tc, RRc, TTc, RTc, Lc)
-            if (cs >= 0) AGX_TERM(cs, Vs, Rs, Ts, RRs, TTs, RTs, Ls)
-#undef AGX_TERM
-            sr = sn;
+            if (rec.s2 >= 0) {
+                double as[6];
+                load6(rec.s2, as);
+                add_term(as, Q, dQ, d2Q, ll1, Vs, Rs, Ts, RRs, TTs, RTs, Ls);
+            }
+        };
+        // Qo holds P_{l-2} and becomes P_l; Qn is P_{l-1}
+        auto recur = [&](const ivec8 &rec, double &Qo, double &dQo, double Qn, double dQn) {
+            const double al = rec_lo(rec), be = rec_hi(rec);
+            if (D1) dQo = fma(al, fma(ct, dQn, -st * Qn), -be * dQo);
+            Qo = fma(al * ct, Qn, -be * Qo);
+        };
+        // Steps j0 .. j1-1 (the first has l = m).  Records past the group are
+        // read (the next group's, or the zero record) but not used.
+        const int j0 = gr.s1, j1 = gr.s2;
+        double cA[6], cB[6];
+        const ivec8 r0 = cld_rec(P.prog, S0 + j0);
+        if (UNI && r0.s1 >= 0) load6(r0.s1, cA);
+        ivec8 r1 = cld_rec(P.prog, S0 + j0 + 1);
+        if (UNI && j0 + 1 < j1 && r1.s1 >= 0) load6(r1.s1, cB);
+        terms(r0, QA, dQA, cA);
+        int j = j0 + 1;
+        for (; j + 1 < j1; j += 2) {
+            const ivec8 r2 = cld_rec(P.prog, S0 + j + 1);
+            if (UNI && r2.s1 >= 0) load6(r2.s1, cA);
+            recur(r1, QB, dQB, QA, dQA);
+            terms(r1, QB, dQB, cB);
+            const ivec8 r3 = cld_rec(P.prog, S0 + j + 2);
+            if (UNI && j + 2 < j1 && r3.s1 >= 0) load6(r3.s1, cB);
+            recur(r2, QA, dQA, QB, dQB);
+            terms(r2, QA, dQA, cA);
+            r1 = r3;
+        }
+        if (j < j1) {
+            recur(r1, QB, dQB, QA, dQA);
+            terms(r1, QB, dQB, cB);
         }
         // ---- apply cos/sin(m phi) to the group sums ------------------------
         Sv = fma(cm, Vc, fma(sm, Vs, Sv));
@@ -298,7 +344,7 @@ __device__ __forceinline__ Where classify(const MPK &P, double x, double y, doub
     w.k = 0;
     w.lnr = 0.0;
     if (!(w.r >= 1.0e-300)) { w.kind = 0; return w; }
-    w.lnr = log(w.r);
+    w.lnr = agx_log(w.r);
     if (w.lnr < P.logr_min) { w.kind = 1; return w; }
     int k = (int)((w.lnr - P.logr_min) * P.inv_dlogr);
     if (k < 0) k = 0;
