@@ -63,6 +63,9 @@ AGX_T_MULTIPOLE, AGX_T_CYLSPLINE = 16, 17
 AGX_PHI, AGX_FORCE, AGX_DERIV = 1, 2, 4
 AGX_OPT_MP_PATH, AGX_OPT_MP_TILE, AGX_OPT_MP_BIN_MIN_N, AGX_OPT_MP_BIN_MIN_TERMS = 1, 2, 3, 4
 AGX_OPT_MP_WAVES = 5
+AGX_OPT_MP_LDS_GRID = 6
+AGX_OPT_MP_BIN_GRID = 7
+AGX_OPT_GRID_PER_CU = 8
 AGX_BATCH_MAX_ITEM = 8192
 AGX_BATCH_LANE, AGX_BATCH_LDS, AGX_BATCH_BINNED = 0, 1, 2
 

@@ -105,9 +105,14 @@ struct PointOut {
 // Agama's gravitational constant, kpc (km/s)^2 / Msol (reference gpu_potential.py:157)
 constexpr double G_AGAMA_D = 4.30091727067736e-06;
 
+// Workgroups per CU of the grid-stride launches (agx_set_option
+// AGX_OPT_GRID_PER_CU; defined in capi.hip).
+extern int g_grid_per_cu;
+
 // Grid size for a grid-stride launch over n items: enough blocks to fill
-// 256 CUs several times over, never more than needed.
-inline unsigned grid_for(int64_t n, int per_cu_blocks = 8)
+// 256 CUs several times over (oversubscribed, so that workgroups finishing
+// early free their slots for the remaining work), never more than needed.
+inline unsigned grid_for(int64_t n, int per_cu_blocks = g_grid_per_cu)
 {
     int64_t want = (n + kBlock - 1) / kBlock;
     int64_t cap = (int64_t)256 * per_cu_blocks;

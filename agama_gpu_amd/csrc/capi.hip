@@ -1,6 +1,10 @@
 // Library-level entry points of the C ABI (version, error text).
 #include "agx_common.h"
 
+namespace agx {
+int g_grid_per_cu = 32;
+}
+
 extern "C" int agx_version(void) { return 10000; }  // 1.0.0
 
 extern "C" const char *agx_error_string(int code)

@@ -94,6 +94,8 @@ static int64_t g_mp_tile = 0;        // 0 = automatic
 static int64_t g_mp_bin_min_n = 200000;
 static int64_t g_mp_bin_min_terms = 24;
 static int g_mp_waves = 0;           // LDS path register cap: 0 auto, 1 never, 4 always
+static int g_mp_lds_grid = 4;        // LDS path: grid = this x resident workgroups
+static int g_mp_bin_grid = 2;        // binned path: grid = this x resident workgroups
 
 struct BinLaunch {
     unsigned grid;
@@ -102,7 +104,9 @@ struct BinLaunch {
 
 static BinLaunch bin_launch(int64_t n)
 {
-    const int64_t max_wg = 256 * 3;   // three 42 KB-LDS workgroups per CU
+    // three 42 KB-LDS workgroups fit per CU; g_mp_bin_grid x that many
+    // workgroups in the grid
+    const int64_t max_wg = 256 * 3 * (int64_t)g_mp_bin_grid;
     int64_t tile = g_mp_tile > 0 ? g_mp_tile : (n + max_wg - 1) / max_wg;
     if (tile < 1024) tile = 1024;
     if (tile > kMaxTile) tile = kMaxTile;
@@ -145,8 +149,15 @@ static void launch_mp(const agx_multipole *mp, const MPK &P, const double *xyz, 
         // potential and force fit 128 VGPRs without spills (4 waves/SIMD);
         // the Hessian does not (it would spill), so it keeps its registers
         const bool cap = g_mp_waves == 4 || (g_mp_waves == 0 && (ORD == ORD_POT || ORD == ORD_GRAD));
+        // Workgroups per CU: g_mp_lds_grid times the number that fit in LDS
+        // at once.  Static partitions over exactly the resident workgroups
+        // end unevenly (the waves sharing a SIMD do not progress at the same
+        // rate: the first workgroups finish 35 % before the last), so the
+        // grid is oversubscribed and the dispatcher hands freed slots to new
+        // workgroups; each one re-stages the table from L2.
         int64_t per_cu = kLdsPerCu / tb;
         if (per_cu > 8) per_cu = 8;
+        per_cu *= g_mp_lds_grid;
         const unsigned grid = grid_for(n, (int)per_cu);
         if (cap)
             hipLaunchKernelGGL((mp_lds_kernel<ORD, 4>), dim3(grid), dim3(kBlock), (size_t)tb, s, P,
@@ -186,6 +197,18 @@ extern "C" int agx_set_option(int key, int64_t value)
     case AGX_OPT_MP_BIN_MIN_TERMS:
         if (value < 0) return AGX_E_SIZE;
         g_mp_bin_min_terms = value;
+        return AGX_OK;
+    case AGX_OPT_GRID_PER_CU:
+        if (value < 1 || value > 64) return AGX_E_SIZE;
+        g_grid_per_cu = (int)value;
+        return AGX_OK;
+    case AGX_OPT_MP_BIN_GRID:
+        if (value < 1 || value > 16) return AGX_E_SIZE;
+        g_mp_bin_grid = (int)value;
+        return AGX_OK;
+    case AGX_OPT_MP_LDS_GRID:
+        if (value < 1 || value > 64) return AGX_E_SIZE;
+        g_mp_lds_grid = (int)value;
         return AGX_OK;
     case AGX_OPT_MP_WAVES:
         if (value != 0 && value != 1 && value != 4) return AGX_E_SIZE;

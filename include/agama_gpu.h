@@ -281,6 +281,13 @@ int agx_leapfrog(const agx_component *d_leaves, int32_t n_leaves,
 #define AGX_OPT_MP_BIN_MIN_TERMS 4
 #define AGX_OPT_MP_WAVES 5   /* LDS path register cap for 4 waves/SIMD: 0 automatic
                                 (potential and force), 1 never, 4 always */
+#define AGX_OPT_MP_LDS_GRID 6 /* LDS path grid: this many times the workgroups that
+                                 fit at once (1..64, default 4) */
+#define AGX_OPT_MP_BIN_GRID 7 /* binned path grid: this many times the workgroups
+                                 that fit at once (1..16, default 2) */
+#define AGX_OPT_GRID_PER_CU 8 /* workgroups per CU of the grid-stride kernels (lane
+                                 Multipole, CylSpline, composite, leapfrog;
+                                 1..64, default 32) */
 int agx_set_option(int key, int64_t value);
 
 /* Library identification and error text. */
