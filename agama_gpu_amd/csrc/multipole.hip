@@ -50,8 +50,8 @@ __global__ __launch_bounds__(kBinThreads) void mp_binned_kernel(
 // One particle per lane with the whole coefficient table staged in LDS once
 // per workgroup: the per-term gathers of the lane kernel become LDS reads.
 // Used when the table is small (<= kLdsTableBytes).
-template <int ORD, int WAVES>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES))) void mp_lds_kernel(MPK P, const double *__restrict__ xyz,
+template <int ORD, int WAVES, int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) void mp_lds_kernel(MPK P, const double *__restrict__ xyz,
                                                         int64_t n, double *__restrict__ phi,
                                                         double *__restrict__ force,
                                                         double *__restrict__ deriv, double inv4piG)
@@ -60,11 +60,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
     const int nv = P.n_int * P.n_lm * 3;               // 16-byte words
     const dvec2 *src = (const dvec2 *)P.poly;
     dvec2 *dst = (dvec2 *)lds_tab;
-    for (int j = threadIdx.x; j < nv; j += kBlock) dst[j] = src[j];
+    for (int j = threadIdx.x; j < nv; j += BLOCK) dst[j] = src[j];
     __syncthreads();
     const int slab_len = P.n_lm * 6;
-    const int64_t stride = (int64_t)gridDim.x * kBlock;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const int64_t stride = (int64_t)gridDim.x * BLOCK;
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) {
         const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
         PointOut o{};
         Where w = classify(P, x, y, z);
@@ -159,12 +159,13 @@ static void launch_mp(const agx_multipole *mp, const MPK &P, const double *xyz, 
         if (per_cu > 8) per_cu = 8;
         per_cu *= g_mp_lds_grid;
         const unsigned grid = grid_for(n, (int)per_cu);
-        if (cap)
-            hipLaunchKernelGGL((mp_lds_kernel<ORD, 4>), dim3(grid), dim3(kBlock), (size_t)tb, s, P,
-                               xyz, n, phi, force, deriv, inv4piG);
-        else
-            hipLaunchKernelGGL((mp_lds_kernel<ORD, 1>), dim3(grid), dim3(kBlock), (size_t)tb, s, P,
-                               xyz, n, phi, force, deriv, inv4piG);
+        if (cap) {
+            hipLaunchKernelGGL((mp_lds_kernel<ORD, 4, kBlock>), dim3(grid), dim3(kBlock), (size_t)tb,
+                               s, P, xyz, n, phi, force, deriv, inv4piG);
+        } else {
+            hipLaunchKernelGGL((mp_lds_kernel<ORD, 1, kBlock>), dim3(grid), dim3(kBlock), (size_t)tb,
+                               s, P, xyz, n, phi, force, deriv, inv4piG);
+        }
     } else if (path == 2) {
         BinLaunch bl = bin_launch(n);
         hipLaunchKernelGGL(mp_binned_kernel<ORD>, dim3(bl.grid), dim3(kBinThreads), 0, s, P, xyz,

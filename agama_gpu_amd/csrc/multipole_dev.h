@@ -104,9 +104,6 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
     const double idl = P.inv_dlogr;
     double s = (lnr - fma((double)k, P.dlogr, P.logr_min)) * idl;
     s = fmin(fmax(s, 0.0), 1.0);
-    const double s2 = s * s, s3 = s2 * s, s4 = s2 * s2, s5 = s4 * s;
-    const double w2 = 2.0 * s, w3 = 3.0 * s2, w4 = 4.0 * s3, w5 = 5.0 * s4;     // C'
-    const double u3 = 6.0 * s, u4 = 12.0 * s2, u5 = 20.0 * s3;                  // C''
 
     auto load6 = [&](int off, double a[6]) {
         if (UNI) {
@@ -119,14 +116,20 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
             a[0] = q0.x; a[1] = q0.y; a[2] = q1.x; a[3] = q1.y; a[4] = q2.x; a[5] = q2.y;
         }
     };
-    auto val = [&](const double a[6]) {
-        return fma(a[5], s5, fma(a[4], s4, fma(a[3], s3, fma(a[2], s2, fma(a[1], s, a[0])))));
-    };
-    auto der = [&](const double a[6]) {
-        return fma(a[5], w5, fma(a[4], w4, fma(a[3], w3, fma(a[2], w2, a[1]))));
-    };
-    auto der2 = [&](const double a[6]) {
-        return fma(a[5], u5, fma(a[4], u4, fma(a[3], u3, 2.0 * a[2])));
+    // C, C' and C'' of the quintic in s by Horner's scheme with synthetic
+    // division: 5 + 4 (+ 3) FMA, and only s itself is held in registers
+    // (powers of s and their derivative weights would take 16-22 VGPRs)
+    auto horner = [&](const double a[6], double &C, double &dC, double &d2C) {
+        double b = a[5], c = 0.0, d = 0.0;
+#pragma unroll
+        for (int q = 4; q >= 0; --q) {
+            if (D2 && q <= 3) d = (q == 3) ? c : fma(d, s, c);
+            if (R1 && q <= 4) c = (q == 4) ? b : fma(c, s, b);
+            b = fma(b, s, a[q]);
+        }
+        C = b;
+        if (R1) dC = c;
+        if (D2) d2C = 2.0 * d;
     };
 
     // ---- angular basis ------------------------------------------------------
@@ -183,15 +186,15 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
         double RRc = 0.0, RRs = 0.0, TTc = 0.0, TTs = 0.0, RTc = 0.0, RTs = 0.0, Lc = 0.0, Ls = 0.0;
         auto add_term = [&](const double a[6], double Q, double dQ, double d2Q, double ll1, double &V,
                             double &Rr, double &Tt, double &RRr, double &TTt, double &RTt, double &LL) {
-            const double C = val(a);
+            double C, dC = 0.0, d2C = 0.0;
+            horner(a, C, dC, d2C);
             V = fma(C, Q, V);
             if (R1) {
-                const double dC = der(a);
                 Rr = fma(dC, Q, Rr);
                 if (D1) Tt = fma(C, dQ, Tt);
                 if (ANG2) RTt = fma(dC, dQ, RTt);
             }
-            if (D2) RRr = fma(der2(a), Q, RRr);
+            if (D2) RRr = fma(d2C, Q, RRr);
             if (ANG2) TTt = fma(C, d2Q, TTt);
             if (ORD == ORD_DENS) LL = fma(ll1 * C, Q, LL);
         };
@@ -281,9 +284,10 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
     // values are not held in registers across it) ----------------------------
     double a0[6];
     load6(0, a0);
-    double C0 = val(a0), dC0 = 0.0, d2C0 = 0.0;
-    if (R1) dC0 = der(a0) * idl;
-    if (D2) d2C0 = der2(a0) * idl * idl;
+    double C0, dC0 = 0.0, d2C0 = 0.0;
+    horner(a0, C0, dC0, d2C0);
+    if (R1) dC0 *= idl;
+    if (D2) d2C0 *= idl * idl;
     double lam = 1.0, mu = 0.0, nu = 0.0;
     if (P.log_scaling) {
         double ex = exp(C0);
