@@ -67,6 +67,15 @@ __device__ __forceinline__ double rcp_nr(double x)
 // 30 instructions against about 90 for the library log, whose extra work
 // handles zero, subnormal, negative and non-finite inputs that do not occur
 // where this is used.
+// An FP64 constant pinned to an SGPR pair: the FMAs then take it as their
+// scalar operand.  Left to itself the compiler materialises each literal FP64
+// constant with two VALU moves into the accumulator register before the FMA.
+__device__ __forceinline__ double sconst(double c)
+{
+    asm volatile("" : "+s"(c));
+    return c;
+}
+
 __device__ __forceinline__ double agx_log(double x)
 {
     double m = __builtin_amdgcn_frexp_mant(x);     // [0.5, 1)
@@ -74,20 +83,45 @@ __device__ __forceinline__ double agx_log(double x)
     if (m < 0.70710678118654752440) { m += m; --e; }
     const double f = (m - 1.0) * rcp_nr(m + 1.0);
     const double f2 = f * f;
-    double p = 1.0 / 21.0;
-    p = fma(p, f2, 1.0 / 19.0);
-    p = fma(p, f2, 1.0 / 17.0);
-    p = fma(p, f2, 1.0 / 15.0);
-    p = fma(p, f2, 1.0 / 13.0);
-    p = fma(p, f2, 1.0 / 11.0);
-    p = fma(p, f2, 1.0 / 9.0);
-    p = fma(p, f2, 1.0 / 7.0);
-    p = fma(p, f2, 1.0 / 5.0);
-    p = fma(p, f2, 1.0 / 3.0);
+    double p = sconst(1.0 / 21.0);
+    p = fma(p, f2, sconst(1.0 / 19.0));
+    p = fma(p, f2, sconst(1.0 / 17.0));
+    p = fma(p, f2, sconst(1.0 / 15.0));
+    p = fma(p, f2, sconst(1.0 / 13.0));
+    p = fma(p, f2, sconst(1.0 / 11.0));
+    p = fma(p, f2, sconst(1.0 / 9.0));
+    p = fma(p, f2, sconst(1.0 / 7.0));
+    p = fma(p, f2, sconst(1.0 / 5.0));
+    p = fma(p, f2, sconst(1.0 / 3.0));
     const double lnm = fma(2.0 * f * f2, p, 2.0 * f);
     const double de = (double)e;
     // ln 2 split so that e * LN2_HI is exact for |e| < 2^11
-    return fma(de, 6.93147180369123816490e-01, fma(de, 1.90821492927058770002e-10, lnm));
+    return fma(de, sconst(6.93147180369123816490e-01), fma(de, sconst(1.90821492927058770002e-10), lnm));
+}
+
+// e^x for |x| < 700, within a few ulp: x = k ln2 + r with |r| <= ln2/2
+// (Cody-Waite reduction, ln 2 in two parts), e^r by its Taylor series to r^13
+// (the next term is below 2^-57), scaled by 2^k.  About 20 VALU instructions.
+__device__ __forceinline__ double agx_exp(double x)
+{
+    const double k = __builtin_rint(x * sconst(1.44269504088896338700e+00));
+    double r = fma(-k, sconst(6.93147180369123816490e-01), x);
+    r = fma(-k, sconst(1.90821492927058770002e-10), r);
+    double p = sconst(1.0 / 6227020800.0);                 // 1/13!
+    p = fma(p, r, sconst(1.0 / 479001600.0));              // 1/12!
+    p = fma(p, r, sconst(1.0 / 39916800.0));
+    p = fma(p, r, sconst(1.0 / 3628800.0));
+    p = fma(p, r, sconst(1.0 / 362880.0));
+    p = fma(p, r, sconst(1.0 / 40320.0));
+    p = fma(p, r, sconst(1.0 / 5040.0));
+    p = fma(p, r, sconst(1.0 / 720.0));
+    p = fma(p, r, sconst(1.0 / 120.0));
+    p = fma(p, r, sconst(1.0 / 24.0));
+    p = fma(p, r, sconst(1.0 / 6.0));
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return __builtin_amdgcn_ldexp(p, (int)k);
 }
 
 constexpr int kBlock = 256;

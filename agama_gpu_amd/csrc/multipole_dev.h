@@ -290,7 +290,7 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
     if (D2) d2C0 *= idl * idl;
     double lam = 1.0, mu = 0.0, nu = 0.0;
     if (P.log_scaling) {
-        double ex = exp(C0);
+        double ex = agx_exp(C0);
         double P0 = rcp_nr(P.invPhi0 - ex);
         double dPdX = P0 * P0 * ex;
         if (D2) d2C0 = dPdX * fma(dC0 * dC0, P0 * (P.invPhi0 + ex), d2C0);
