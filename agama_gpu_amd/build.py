@@ -32,6 +32,11 @@ HIPCC_FLAGS = [
     # kernels that costs 25-60 VGPRs per kernel (and scratch spills where a
     # register cap applies).  IR-level LICM still hoists the real work.
     "-mllvm", "-disable-machine-licm",
+    # Sinking common stores out of the branches of a point's evaluation turns
+    # them into one store through a phi of two stack addresses, which keeps
+    # the output struct in scratch memory (a scratch store and reload per
+    # particle); without the sinking it stays in registers.
+    "-mllvm", "-simplifycfg-sink-common=false",
     f"-I{INCLUDE}", f"-I{CSRC}",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]

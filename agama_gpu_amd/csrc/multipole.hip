@@ -57,15 +57,23 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) 
                                                         double *__restrict__ deriv, double inv4piG)
 {
     extern __shared__ __align__(16) double lds_tab[];
+    // Positions are read one grid-stride iteration ahead (the first before
+    // the table staging), so the HBM latency of the next particle overlaps
+    // the arithmetic of the current one.
+    const int64_t stride = (int64_t)gridDim.x * BLOCK;
+    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    double xn = 0.0, yn = 0.0, zn = 0.0;
+    if (i < n) { xn = xyz[3 * i]; yn = xyz[3 * i + 1]; zn = xyz[3 * i + 2]; }
     const int nv = P.n_int * P.n_lm * 3;               // 16-byte words
     const dvec2 *src = (const dvec2 *)P.poly;
     dvec2 *dst = (dvec2 *)lds_tab;
     for (int j = threadIdx.x; j < nv; j += BLOCK) dst[j] = src[j];
     __syncthreads();
     const int slab_len = P.n_lm * 6;
-    const int64_t stride = (int64_t)gridDim.x * BLOCK;
-    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) {
-        const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+    for (; i < n; i += stride) {
+        const double x = xn, y = yn, z = zn;
+        const int64_t inx = i + stride;
+        if (inx < n) { xn = xyz[3 * inx]; yn = xyz[3 * inx + 1]; zn = xyz[3 * inx + 2]; }
         PointOut o{};
         Where w = classify(P, x, y, z);
         if (w.kind == 2)
