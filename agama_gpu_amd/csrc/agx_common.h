@@ -126,6 +126,24 @@ __device__ __forceinline__ double agx_exp(double x)
 
 constexpr int kBlock = 256;
 
+// Grid-stride loop over points i < n of an AoS (n,3) position array, calling
+// body(i, x, y, z); each position is read one iteration ahead, so its HBM
+// latency overlaps the previous point's arithmetic.
+template <int BLOCK, class Body>
+__device__ __forceinline__ void for_each_point(const double *__restrict__ xyz, int64_t n, Body body)
+{
+    const int64_t stride = (int64_t)gridDim.x * BLOCK;
+    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    double xn = 0.0, yn = 0.0, zn = 0.0;
+    if (i < n) { xn = xyz[3 * i]; yn = xyz[3 * i + 1]; zn = xyz[3 * i + 2]; }
+    for (; i < n; i += stride) {
+        const double x = xn, y = yn, z = zn;
+        const int64_t inx = i + stride;
+        if (inx < n) { xn = xyz[3 * inx]; yn = xyz[3 * inx + 1]; zn = xyz[3 * inx + 2]; }
+        body(i, x, y, z);
+    }
+}
+
 // Derivative order of an evaluation: potential; +gradient; +Hessian; density.
 enum { ORD_POT = 0, ORD_GRAD = 1, ORD_HESS = 2, ORD_DENS = 3 };
 

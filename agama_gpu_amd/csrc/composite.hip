@@ -34,9 +34,7 @@ __global__ __launch_bounds__(kBlock) void composite_kernel(CompositeArgs A,
 {
     constexpr bool G = ORD == ORD_GRAD || ORD == ORD_HESS;
     constexpr bool H = ORD == ORD_HESS;
-    const int64_t stride = (int64_t)gridDim.x * kBlock;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-        const double x0 = xyz[3 * i], y0 = xyz[3 * i + 1], z0 = xyz[3 * i + 2];
+    for_each_point<kBlock>(xyz, n, [&](int64_t i, double x0, double y0, double z0) {
         double P = 0.0, g[3] = {0.0, 0.0, 0.0}, h[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         for (int c = 0; c < A.n; ++c) {
             const agx_component &C = A.c[c];
@@ -46,7 +44,7 @@ __global__ __launch_bounds__(kBlock) void composite_kernel(CompositeArgs A,
         if (G && force) { force[3 * i] = -g[0]; force[3 * i + 1] = -g[1]; force[3 * i + 2] = -g[2]; }
         if (H && deriv)
             for (int k = 0; k < 6; ++k) deriv[6 * i + k] = -h[k];
-    }
+    });
 }
 
 static int check_components(const agx_component *comps, int n_comps, CompositeArgs &A)

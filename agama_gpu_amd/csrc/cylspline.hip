@@ -9,17 +9,16 @@ __global__ __launch_bounds__(kBlock) void cyl_kernel(CSK P, const double *__rest
                                                      double *__restrict__ force,
                                                      double *__restrict__ deriv)
 {
-    const int64_t stride = (int64_t)gridDim.x * kBlock;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    for_each_point<kBlock>(xyz, n, [&](int64_t i, double x, double y, double z) {
         PointOut o{};
-        cylspline_point<ORD>(P, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], o);
+        cylspline_point<ORD>(P, x, y, z, o);
         if (phi) phi[i] = o.phi;
         if ((ORD == ORD_GRAD || ORD == ORD_HESS) && force) {
             force[3 * i] = -o.g[0]; force[3 * i + 1] = -o.g[1]; force[3 * i + 2] = -o.g[2];
         }
         if (ORD == ORD_HESS && deriv)
             for (int k = 0; k < 6; ++k) deriv[6 * i + k] = -o.h[k];
-    }
+    });
 }
 
 static int check_cs(const agx_cylspline *cs, const double *xyz, int64_t n)

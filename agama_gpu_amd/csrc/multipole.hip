@@ -10,9 +10,7 @@ __global__ __launch_bounds__(kBlock) void mp_lane_kernel(MPK P, const double *__
                                                          double *__restrict__ force,
                                                          double *__restrict__ deriv, double inv4piG)
 {
-    const int64_t stride = (int64_t)gridDim.x * kBlock;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-        const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+    for_each_point<kBlock>(xyz, n, [&](int64_t i, double x, double y, double z) {
         PointOut o{};
         Where w = classify(P, x, y, z);
         if (w.kind == 2)
@@ -21,7 +19,7 @@ __global__ __launch_bounds__(kBlock) void mp_lane_kernel(MPK P, const double *__
         else
             extrap<ORD>(P, w, x, y, z, o, inv4piG);
         store<ORD>(o, i, phi, force, deriv);
-    }
+    });
 }
 
 template <int ORD>
