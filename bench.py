@@ -9,8 +9,14 @@ tables are resident in HBM before the timed region; one step = one launch
 over all N particles.
 
 Prints one JSON line (rank 0).  Extra keys: ``roofline`` (FP64 VALU bound,
-algorithmic flops per launch / mean kernel time) and ``cpu_baseline`` (the
-oracle's OpenMP C restatement on this host's cores, same workload).
+algorithmic flops per launch / mean kernel time; HBM traffic per launch from
+the committed rocprof summary) and ``cpu_baseline`` (the oracle's OpenMP C
+restatement on this host's cores, same workload), and the other legs of the
+metric (skipped with --no-extras): ``dense`` (the same points through the
+dense 81-term lmax-8 FIRE table), ``analytic`` (NFW, HBM-bound), ``batched``
+(256-potential ragged ensemble, one launch vs one per potential), ``grad``
+(force-fit forward+backward) and ``leapfrog`` (HIP-graph replay vs per-step
+API calls).
 """
 from __future__ import annotations
 
@@ -152,6 +158,50 @@ def bench_batched(dev, reps: int = 5):
             "setup_s": t_tune - t_setup, "tune_s": t_ready - t_tune}
 
 
+def _event_times(fn, reps: int = 10):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(reps)]
+    for a, b in evs:
+        a.record()
+        fn()
+        b.record()
+    torch.cuda.synchronize()
+    return [a.elapsed_time(b) for a, b in evs]
+
+
+def bench_dense(dev, x_np, xyz, phi, force):
+    """The same N=10M points through the dense lmax-8 table of the reference's
+    FIRE fixture (600.dark.none_8.coef_mul_DR: all 81 harmonics, 25 nodes),
+    pot+force: the table the FP64-VALU roofline target is stated for."""
+    from agama_gpu_amd import MultipolePotentialGPU, read_coefs
+    coefs = read_coefs(os.path.join(ROOT, "tests", "golden", "600.dark.none_8.coef_mul_DR"))
+    pot = MultipolePotentialGPU(coefs, device=dev)
+    kinds = classify(pot._data, x_np)
+    flops, info = opcount_multipole(pot._data, kinds, order=1)
+    ts = _event_times(lambda: pot.eval_into(xyz, phi=phi, force=force))
+    ms = float(np.median(ts))
+    ach = flops / (ms * 1e-3) / 1e12
+    return {"table": "600.dark.none_8.coef_mul_DR (81 terms, 25 nodes)", "kernel_ms_median": ms,
+            "particles_per_s": len(x_np) / (ms * 1e-3), "achieved_tflops": ach,
+            "frac_fp64_peak": ach / PEAK_FP64_TFLOPS, "opcount": info}
+
+
+def bench_analytic(dev, xyz, phi, force):
+    """NFW (M=1e12, rs=20 kpc) pot+force at the same points: the HBM-bound
+    member of the path (56 B per particle)."""
+    from agama_gpu_amd import NFWPotentialGPU
+    pot = NFWPotentialGPU(mass=1e12, scaleRadius=20.0, device=dev)
+    n = xyz.shape[0]
+    ts = _event_times(lambda: pot.eval_into(xyz, phi=phi, force=force))
+    ms = float(np.median(ts))
+    gbs = 56.0 * n / (ms * 1e-3) / 1e9
+    return {"kernel_ms_median": ms, "particles_per_s": n / (ms * 1e-3), "achieved_gbs": gbs,
+            "frac_hbm_peak": gbs / PEAK_HBM_GBS}
+
+
 def bench_grad(dev, reps: int = 5):
     """Force fit: Multipole lmax 8 (2 x 81 x 50 live table entries) shifted
     along a 200-knot spline centre, N=1M tracers; one step = loss (mean
@@ -232,7 +282,7 @@ def main():
                     help="seconds of untimed steps before the warm-up steps (GPU clock ramp)")
     ap.add_argument("--n", type=int, default=10_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-ensemble and gradient legs")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -301,6 +351,7 @@ def main():
     ms_step = (t1 - t0) * 1e3 / args.steps
     kern_ms = [a.elapsed_time(b) for a, b in evs]
     k_mean = float(np.mean(kern_ms))
+    k_median = float(np.median(kern_ms))
     value = N / (ms_step * 1e-3)
     achieved = flops / (k_mean * 1e-3) / 1e12
 
@@ -309,7 +360,8 @@ def main():
         from oracle.oracle import OracleMultipole
         orc = OracleMultipole.from_coefs(coefs)
         th = cpu_threads()
-        orc.pot_force(x_np[:100_000], th)
+        for _ in range(2):
+            orc.pot_force(x_np, th)
         ts = []
         for _ in range(args.cpu_reps):
             c0 = time.perf_counter()
@@ -317,15 +369,18 @@ def main():
             ts.append(time.perf_counter() - c0)
         t_cpu = float(np.median(ts))
         cpu = {"value": N / t_cpu, "unit": "particles/s", "cores": th, "kind": "port",
-               "sample": f"full N={N} workload, median of {args.cpu_reps} after 1 warmup; "
+               "sample": f"full N={N} workload, median of {args.cpu_reps} after 2 warmups; "
                          f"oracle/mp_oracle.c -O3 OpenMP on {cpu_model()}"}
 
     extras = {}
     if not args.no_extras:
+        extras["dense"] = bench_dense(dev, x_np, xyz, phi, force)
+        extras["analytic"] = bench_analytic(dev, xyz, phi, force)
         extras["batched"] = bench_batched(dev)
         extras["grad"] = bench_grad(dev)
         extras["leapfrog"] = bench_leapfrog(dev)
 
+    traffic = profiled_traffic("multipole_halo_lmax8_pot_force_N10M.md")
     line = {
         "metric": "FP64 particles/s (Multipole lmax=8, pot+force, N=10M); batched+grad vs plain",
         "value": value,
@@ -346,9 +401,11 @@ def main():
                    "points": "Hernquist a=20 seed 0, 1% log-uniform [0.01,2000]"},
         "roofline": {"bound": "valu_fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
-                     "traffic": profiled_traffic("multipole_halo_lmax8_pot_force_N10M.md"),
+                     "traffic": traffic,
                      "algorithmic_bytes": 56 * N,
-                     "kernel_ms": k_mean, "flops_per_launch": flops, "opcount": opinfo},
+                     "kernel_ms": k_mean, "kernel_ms_median": k_median,
+                     "hbm_gbs": (traffic if traffic else 56 * N) / (k_mean * 1e-3) / 1e9,
+                     "flops_per_launch": flops, "opcount": opinfo},
         "cpu_baseline": cpu,
         **extras,
     }
