@@ -62,12 +62,11 @@ AGX_T_MULTIPOLE, AGX_T_CYLSPLINE = 16, 17
 
 AGX_PHI, AGX_FORCE, AGX_DERIV = 1, 2, 4
 AGX_OPT_MP_PATH, AGX_OPT_MP_TILE, AGX_OPT_MP_BIN_MIN_N, AGX_OPT_MP_BIN_MIN_TERMS = 1, 2, 3, 4
-AGX_OPT_MP_WAVES = 5
 AGX_OPT_MP_LDS_GRID = 6
 AGX_OPT_MP_BIN_GRID = 7
 AGX_OPT_GRID_PER_CU = 8
-AGX_BATCH_MAX_ITEM = 8192
-AGX_BATCH_LANE, AGX_BATCH_LDS, AGX_BATCH_BINNED = 0, 1, 2
+AGX_BATCH_MAX_ITEM = 16384
+AGX_BATCH_LANE, AGX_BATCH_LDS = 0, 1
 
 # agx_sched_entry as a numpy record (int32 leaf, pad; ampl, scale, center[3])
 SCHED_ENTRY_DTYPE = np.dtype([("leaf", "<i4"), ("pad", "<i4"), ("ampl", "<f8"), ("scale", "<f8"),
@@ -111,7 +110,6 @@ _SIGNATURES = {
     "agx_batch_plan": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
     "agx_batch_lds_table_doubles": (C.c_int64, []),
-    "agx_batch_max_bins": (C.c_int32, []),
     "agx_leapfrog": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32,
                                C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                C.c_int64, C.c_double, C.c_int32, C.c_void_p]),

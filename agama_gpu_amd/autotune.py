@@ -1,19 +1,18 @@
 """Per-lmax launch tuning for the batched Multipole kernel.
 
 For a potential class -- lmax, active harmonics, and whether its table fits
-LDS and the binned sort -- the tuner times the batched kernel on that potential alone for every feasible
-(path, particles per work item) pair:
+LDS -- the tuner times the batched kernel on that potential alone for every
+feasible (path, particles per work item) pair:
 
   path  AGX_BATCH_LANE    per-lane coefficient reads from L2
         AGX_BATCH_LDS     whole table staged in LDS (only if it fits)
-        AGX_BATCH_BINNED  items sorted by radial interval (n_int + 1 <= agx_batch_max_bins())
-  rows per item  2048, 4096, 8192
+  rows per item  2048, 4096, 8192, 16384
 
 and keeps the fastest.  Results are cached in memory and, when
 ``AGAMA_GPU_TUNE_CACHE`` names a file, in that JSON file, keyed by the GPU
 architecture and the shape.  Without tuning, ``default_choice`` gives the
-rule the measurements on gfx950 support: LDS staging when the table fits and
-has fewer than 24 terms, binning from 24 terms, 4096 rows per item.
+rule the measurements on gfx950 support: LDS staging whenever the table fits,
+8192 rows per item.
 """
 from __future__ import annotations
 
@@ -26,16 +25,15 @@ import numpy as np
 
 from . import _lib
 
-ITEM_ROWS = (2048, 4096, 8192)
+ITEM_ROWS = (2048, 4096, 8192, 16384)
 _cache: Dict[str, Tuple[int, int]] = {}
 _lock = threading.Lock()
 
 
-def shape_key(pot) -> Tuple[int, int, int, int]:
-    """Tuning class: lmax, active harmonics and which paths the table fits
-    (potentials of one class share a choice; the grid length enters only
-    through the fit flags)."""
-    return int(pot.lmax), int(pot.n_lm), int(lds_fits(pot)), int(bin_fits(pot))
+def shape_key(pot) -> Tuple[int, int, int]:
+    """Tuning class: lmax, active harmonics and whether the table fits LDS
+    (potentials of one class share a choice)."""
+    return int(pot.lmax), int(pot.n_lm), int(lds_fits(pot))
 
 
 def _arch(device) -> str:
@@ -50,15 +48,9 @@ def lds_fits(pot) -> bool:
     return pot.n_intervals * pot.n_lm * 6 <= int(_lib.load().agx_batch_lds_table_doubles())
 
 
-def bin_fits(pot) -> bool:
-    return pot.n_intervals + 1 <= int(_lib.load().agx_batch_max_bins())
-
-
 def default_choice(pot) -> Tuple[int, int]:
-    if lds_fits(pot) and pot.n_lm < 24:
-        return _lib.AGX_BATCH_LDS, 4096
-    if bin_fits(pot) and pot.n_lm >= 24:
-        return _lib.AGX_BATCH_BINNED, 8192
+    if lds_fits(pot):
+        return _lib.AGX_BATCH_LDS, 8192
     return _lib.AGX_BATCH_LANE, 4096
 
 
@@ -68,8 +60,6 @@ def candidates(pot):
         out.append((_lib.AGX_BATCH_LANE, rows))
         if lds_fits(pot):
             out.append((_lib.AGX_BATCH_LDS, rows))
-        if bin_fits(pot):
-            out.append((_lib.AGX_BATCH_BINNED, rows))
     return out
 
 

@@ -2,22 +2,18 @@
 // launch.
 //
 // The host plan (agx_batch_plan) cuts every potential's particle range into
-// work items of at most 8192 particles and orders them by estimated cost,
-// largest first.  One workgroup per resident slot of the chip loops: it takes
-// the next item from a device counter (one atomic per item), loads that
-// potential's descriptor with scalar loads, and evaluates the item by the
-// path the plan chose for that potential:
-//   lane   one particle per lane, coefficients gathered from L2
-//   lds    the potential's whole table staged into LDS, then one particle
-//          per lane reading coefficients from LDS
-//   binned the item is one binned tile (counting sort by radial interval in
-//          LDS, wavefront-uniform interval, scalar coefficient loads)
-// The LDS of the tile sort and the staged table share one buffer, so the
-// kernel keeps the occupancy of the single-potential kernels.  Items whose
-// potential does not fit the chosen path (table too large for LDS, too many
-// radial intervals or too many particles for a tile) fall back to the lane
-// path on the device.  No reference counterpart: the reference evaluates one
-// potential per launch (gpu_potential.py:757-793).
+// work items of at most AGX_BATCH_MAX_ITEM particles and orders them by
+// estimated cost, largest first.  One 1024-thread workgroup per CU loops: it
+// takes the next item from a device counter (one atomic per item), loads that
+// potential's descriptor with scalar loads, and evaluates the item by the path
+// the plan chose for that potential:
+//   lane  one particle per lane, coefficients gathered from L2
+//   lds   the potential's whole table staged into LDS (up to ~160 KB), then
+//         one particle per lane reading coefficients from LDS
+// One workgroup of 16 wavefronts per CU owns the CU's whole LDS, so any table
+// of the single-potential LDS path fits; an item whose table does not fit
+// falls back to the lane path on the device.  No reference counterpart: the
+// reference evaluates one potential per launch (gpu_potential.py:757-793).
 #include <algorithm>
 #include <vector>
 
@@ -25,41 +21,52 @@
 
 namespace agx {
 
-// The tile sort of the batch kernel takes at most 96 bins (n_int <= 95), so
-// that its LDS (about 37 KB) lets four 256-thread workgroups share a CU.
-using BatchBin = BinLDST<kMaxTile, 96>;
-constexpr int kBatchTableDoubles = (int)(sizeof(BatchBin) / sizeof(double));
-
-union BatchLDS {
-    BatchBin bin;
-    double tab[kBatchTableDoubles];
-};
+constexpr int kBatchBlock = 1024;
+// dynamic LDS for the table: the CU's 160 KB less room for the static
+// item index
+constexpr int kBatchTableDoubles = (160 * 1024 - 256) / 8;
 
 __global__ void batch_queue_reset(int *q) { if (threadIdx.x == 0) q[0] = 0; }
 
+// rows [start, start + count) with the table at tab_base; each thread reads
+// its next position one iteration ahead
 template <int ORD>
-__device__ __forceinline__ void lane_point(const MPK &P, const double *tab_base, const double *xyz,
-                                           int64_t i, double *phi, double *force, double *deriv)
+__device__ __forceinline__ void batch_rows(const MPK &P, const double *tab_base,
+                                           const double *__restrict__ xyz, int64_t start, int count,
+                                           double *phi, double *force, double *deriv)
 {
-    const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
-    PointOut o{};
-    Where w = classify(P, x, y, z);
-    if (w.kind == 2)
-        interior<ORD, false>(P, tab_base + (int64_t)w.k * P.n_lm * 6, x, y, z, w.r2, w.r, w.lnr,
-                             w.k, o, 0.0);
-    else
-        extrap<ORD>(P, w, x, y, z, o, 0.0);
-    store<ORD>(o, i, phi, force, deriv);
+    int j = threadIdx.x;
+    double xn = 0.0, yn = 0.0, zn = 0.0;
+    if (j < count) {
+        const int64_t i = start + j;
+        xn = xyz[3 * i]; yn = xyz[3 * i + 1]; zn = xyz[3 * i + 2];
+    }
+    for (; j < count; j += kBatchBlock) {
+        const double x = xn, y = yn, z = zn;
+        const int jn = j + kBatchBlock;
+        if (jn < count) {
+            const int64_t in = start + jn;
+            xn = xyz[3 * in]; yn = xyz[3 * in + 1]; zn = xyz[3 * in + 2];
+        }
+        PointOut o{};
+        Where w = classify(P, x, y, z);
+        if (w.kind == 2)
+            interior<ORD, false>(P, tab_base + (int64_t)w.k * P.n_lm * 6, x, y, z, w.r2, w.r,
+                                 w.lnr, w.k, o, 0.0);
+        else
+            extrap<ORD>(P, w, x, y, z, o, 0.0);
+        store<ORD>(o, start + j, phi, force, deriv);
+    }
 }
 
 template <int ORD>
-__global__ __launch_bounds__(kBinThreads) void mp_batch_kernel(
+__global__ __launch_bounds__(kBatchBlock) void mp_batch_kernel(
     const char *__restrict__ descs, int64_t stride, int n_pot,
     const agx_batch_item *__restrict__ items, int64_t n_items, const double *__restrict__ xyz,
     int64_t n_rows, double *__restrict__ phi, double *__restrict__ force,
     double *__restrict__ deriv, int *__restrict__ queue)
 {
-    __shared__ BatchLDS U;
+    extern __shared__ __align__(16) double tab[];
     __shared__ int item_s;
     const int tid = threadIdx.x;
     for (;;) {
@@ -75,34 +82,15 @@ __global__ __launch_bounds__(kBinThreads) void mp_batch_kernel(
         if (start < 0 || count < 1 || start > n_rows - count || pot < 0 || pot >= n_pot) continue;
         const MPK P = load_desc<MPK>(descs + (int64_t)pot * stride);
         const int tab_len = P.n_int * P.n_lm * 6;
-        if (path == AGX_BATCH_BINNED && P.n_int + 1 <= BatchBin::kBins && count <= BatchBin::kTile) {
-            binned_tile(P, xyz, start, count, U.bin, [&](int b, bool valid, int64_t i) {
-                if (!valid) return;
-                const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
-                PointOut o{};
-                if (b > 0) {
-                    const int k = b - 1;
-                    const double r2 = fma(x, x, fma(y, y, z * z));
-                    const double r = sqrt(r2);
-                    interior<ORD, true>(P, P.poly + (int64_t)k * P.n_lm * 6, x, y, z, r2, r, agx_log(r),
-                                        k, o, 0.0);
-                } else {
-                    Where w = classify(P, x, y, z);
-                    extrap<ORD>(P, w, x, y, z, o, 0.0);
-                }
-                store<ORD>(o, i, phi, force, deriv);
-            });
-        } else if (path == AGX_BATCH_LDS && tab_len <= kBatchTableDoubles) {
+        if (path == AGX_BATCH_LDS && tab_len <= kBatchTableDoubles) {
             const dvec2 *src = (const dvec2 *)P.poly;
-            dvec2 *dst = (dvec2 *)U.tab;
-            for (int j = tid; j < tab_len / 2; j += kBinThreads) dst[j] = src[j];
+            dvec2 *dst = (dvec2 *)tab;
+            for (int j = tid; j < tab_len / 2; j += kBatchBlock) dst[j] = src[j];
             __syncthreads();
-            for (int j = tid; j < count; j += kBinThreads)
-                lane_point<ORD>(P, U.tab, xyz, start + j, phi, force, deriv);
+            batch_rows<ORD>(P, tab, xyz, start, count, phi, force, deriv);
             __syncthreads();
         } else {
-            for (int j = tid; j < count; j += kBinThreads)
-                lane_point<ORD>(P, P.poly, xyz, start + j, phi, force, deriv);
+            batch_rows<ORD>(P, P.poly, xyz, start, count, phi, force, deriv);
         }
     }
 }
@@ -112,6 +100,7 @@ static unsigned batch_grid(int64_t n_items)
 {
     static int per_cu[4] = {0, 0, 0, 0};
     static int n_cu = 0;
+    const size_t lds = (size_t)kBatchTableDoubles * sizeof(double);
     if (!n_cu) {
         int dev = 0, v = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
@@ -121,8 +110,10 @@ static unsigned batch_grid(int64_t n_items)
             n_cu = 256;
     }
     if (!per_cu[ORD]) {
+        hipFuncSetAttribute((const void *)mp_batch_kernel<ORD>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         int b = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, mp_batch_kernel<ORD>, kBinThreads, 0) !=
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, mp_batch_kernel<ORD>, kBatchBlock, lds) !=
                 hipSuccess || b < 1)
             b = 1;
         per_cu[ORD] = b;
@@ -139,7 +130,6 @@ using namespace agx;
 
 extern "C" int64_t agx_batch_lds_table_doubles(void) { return kBatchTableDoubles; }
 
-extern "C" int32_t agx_batch_max_bins(void) { return BatchBin::kBins; }
 
 extern "C" int agx_batch_plan(int32_t n_pot, const int64_t *counts, const int32_t *chunk,
                               const int32_t *path, const double *cost, agx_batch_item *items,
@@ -150,7 +140,7 @@ extern "C" int agx_batch_plan(int32_t n_pot, const int64_t *counts, const int32_
     int64_t total = 0;
     for (int32_t p = 0; p < n_pot; ++p) {
         if (counts[p] < 0 || chunk[p] < 1 || chunk[p] > AGX_BATCH_MAX_ITEM) return AGX_E_SIZE;
-        if (path[p] < AGX_BATCH_LANE || path[p] > AGX_BATCH_BINNED) return AGX_E_WHAT;
+        if (path[p] < AGX_BATCH_LANE || path[p] > AGX_BATCH_LDS) return AGX_E_WHAT;
         total += (counts[p] + chunk[p] - 1) / chunk[p];
     }
     *n_items = total;
@@ -205,15 +195,15 @@ extern "C" int agx_multipole_batch_eval(const void *d_desc, int64_t desc_stride,
     hipLaunchKernelGGL(batch_queue_reset, dim3(1), dim3(64), 0, s, (int *)d_queue);
     if (deriv)
         hipLaunchKernelGGL(mp_batch_kernel<ORD_HESS>, dim3(batch_grid<ORD_HESS>(n_items)),
-                           dim3(kBinThreads), 0, s, descs, desc_stride, n_pot, d_items, n_items,
+                           dim3(kBatchBlock), kBatchTableDoubles * sizeof(double), s, descs, desc_stride, n_pot, d_items, n_items,
                            d_xyz, n_rows, phi, force, deriv, (int *)d_queue);
     else if (force)
         hipLaunchKernelGGL(mp_batch_kernel<ORD_GRAD>, dim3(batch_grid<ORD_GRAD>(n_items)),
-                           dim3(kBinThreads), 0, s, descs, desc_stride, n_pot, d_items, n_items,
+                           dim3(kBatchBlock), kBatchTableDoubles * sizeof(double), s, descs, desc_stride, n_pot, d_items, n_items,
                            d_xyz, n_rows, phi, force, nullptr, (int *)d_queue);
     else
         hipLaunchKernelGGL(mp_batch_kernel<ORD_POT>, dim3(batch_grid<ORD_POT>(n_items)),
-                           dim3(kBinThreads), 0, s, descs, desc_stride, n_pot, d_items, n_items,
+                           dim3(kBatchBlock), kBatchTableDoubles * sizeof(double), s, descs, desc_stride, n_pot, d_items, n_items,
                            d_xyz, n_rows, phi, nullptr, nullptr, (int *)d_queue);
     return hipGetLastError() == hipSuccess ? AGX_OK : AGX_E_LAUNCH;
 }

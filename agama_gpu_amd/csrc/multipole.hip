@@ -47,9 +47,10 @@ __global__ __launch_bounds__(kBinThreads) void mp_binned_kernel(
 
 // One particle per lane with the whole coefficient table staged in LDS once
 // per workgroup: the per-term gathers of the lane kernel become LDS reads.
-// Used when the table is small (<= kLdsTableBytes).
-template <int ORD, int WAVES, int BLOCK>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) void mp_lds_kernel(MPK P, const double *__restrict__ xyz,
+// Used for tables of up to 160 KB (the LDS of one CU); the workgroup size is
+// chosen so that the workgroups that fit in LDS hold 16 wavefronts per CU.
+template <int ORD, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void mp_lds_kernel(MPK P, const double *__restrict__ xyz,
                                                         int64_t n, double *__restrict__ phi,
                                                         double *__restrict__ force,
                                                         double *__restrict__ deriv, double inv4piG)
@@ -99,7 +100,6 @@ static int g_mp_path = 0;            // 0 auto, 1 lane, 2 binned, 3 LDS-staged l
 static int64_t g_mp_tile = 0;        // 0 = automatic
 static int64_t g_mp_bin_min_n = 200000;
 static int64_t g_mp_bin_min_terms = 24;
-static int g_mp_waves = 0;           // LDS path register cap: 0 auto, 1 never, 4 always
 static int g_mp_lds_grid = 4;        // LDS path: grid = this x resident workgroups
 static int g_mp_bin_grid = 2;        // binned path: grid = this x resident workgroups
 
@@ -133,12 +133,33 @@ static bool binned_auto(const agx_multipole *mp, int64_t n)
     return mp->n_int + 1 <= kMaxBins && n >= g_mp_bin_min_n && mp->n_lm >= g_mp_bin_min_terms;
 }
 
-constexpr int64_t kLdsTableBytes = 64 * 1024;
 constexpr int64_t kLdsPerCu = 160 * 1024;
+constexpr int64_t kLdsTableBytes = kLdsPerCu;
 
 static int64_t table_bytes(const agx_multipole *mp)
 {
     return (int64_t)mp->n_int * mp->n_lm * 6 * (int64_t)sizeof(double);
+}
+
+static unsigned lds_grid(int64_t n, int block, int64_t per_cu)
+{
+    int64_t want = (n + block - 1) / block, cap = 256 * per_cu;
+    if (want > cap) want = cap;
+    if (want < 1) want = 1;
+    return (unsigned)want;
+}
+
+// Dynamic LDS beyond 64 KB has to be allowed per kernel.
+template <int ORD, int BLOCK>
+static void lds_attr(int64_t bytes)
+{
+    static int64_t set = 0;
+    if (bytes > set) {
+        if (hipFuncSetAttribute((const void *)mp_lds_kernel<ORD, BLOCK>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsPerCu) ==
+            hipSuccess)
+            set = kLdsPerCu;
+    }
 }
 
 template <int ORD>
@@ -148,29 +169,38 @@ static void launch_mp(const agx_multipole *mp, const MPK &P, const double *xyz, 
     const int64_t tb = table_bytes(mp);
     const bool lds_ok = tb <= kLdsTableBytes;
     int path = g_mp_path;
-    if (path == 0) path = binned_auto(mp, n) ? 2 : (lds_ok ? 3 : 1);
+    // automatic: the LDS-staged table whenever it fits (and N is not tiny),
+    // else binning for many terms and large N, else per-lane table reads
+    if (path == 0) path = (lds_ok && n >= 4096) ? 3 : (binned_auto(mp, n) ? 2 : 1);
     if (path == 3 && !lds_ok) path = 1;
     if (path == 2 && mp->n_int + 1 > kMaxBins) path = 1;
     if (path == 3) {
-        // potential and force fit 128 VGPRs without spills (4 waves/SIMD);
-        // the Hessian does not (it would spill), so it keeps its registers
-        const bool cap = g_mp_waves == 4 || (g_mp_waves == 0 && (ORD == ORD_POT || ORD == ORD_GRAD));
-        // Workgroups per CU: g_mp_lds_grid times the number that fit in LDS
-        // at once.  Static partitions over exactly the resident workgroups
-        // end unevenly (the waves sharing a SIMD do not progress at the same
-        // rate: the first workgroups finish 35 % before the last), so the
-        // grid is oversubscribed and the dispatcher hands freed slots to new
+        // Workgroups per CU that fit in LDS, and a workgroup size that gives
+        // them 16 wavefronts together (4 per SIMD): 256 threads for tables
+        // up to 40 KB, 512 up to 80 KB, 1024 up to 160 KB.
+        int64_t fit = kLdsPerCu / tb;
+        if (fit > 8) fit = 8;
+        // The grid is g_mp_lds_grid times the workgroups that fit at once:
+        // static partitions over exactly the resident workgroups end unevenly
+        // (the waves sharing a SIMD do not progress at the same rate: the
+        // first workgroups finish 35 % before the last), so the grid is
+        // oversubscribed and the dispatcher hands freed slots to new
         // workgroups; each one re-stages the table from L2.
-        int64_t per_cu = kLdsPerCu / tb;
-        if (per_cu > 8) per_cu = 8;
-        per_cu *= g_mp_lds_grid;
-        const unsigned grid = grid_for(n, (int)per_cu);
-        if (cap) {
-            hipLaunchKernelGGL((mp_lds_kernel<ORD, 4, kBlock>), dim3(grid), dim3(kBlock), (size_t)tb,
-                               s, P, xyz, n, phi, force, deriv, inv4piG);
+        const int64_t per_cu = fit * g_mp_lds_grid;
+        if (fit >= 4) {
+            const unsigned grid = grid_for(n, (int)per_cu);
+            hipLaunchKernelGGL((mp_lds_kernel<ORD, 256>), dim3(grid), dim3(256), (size_t)tb, s, P,
+                               xyz, n, phi, force, deriv, inv4piG);
+        } else if (fit >= 2) {
+            const unsigned grid = lds_grid(n, 512, per_cu);
+            lds_attr<ORD, 512>(tb);
+            hipLaunchKernelGGL((mp_lds_kernel<ORD, 512>), dim3(grid), dim3(512), (size_t)tb, s, P,
+                               xyz, n, phi, force, deriv, inv4piG);
         } else {
-            hipLaunchKernelGGL((mp_lds_kernel<ORD, 1, kBlock>), dim3(grid), dim3(kBlock), (size_t)tb,
-                               s, P, xyz, n, phi, force, deriv, inv4piG);
+            const unsigned grid = lds_grid(n, 1024, per_cu);
+            lds_attr<ORD, 1024>(tb);
+            hipLaunchKernelGGL((mp_lds_kernel<ORD, 1024>), dim3(grid), dim3(1024), (size_t)tb, s, P,
+                               xyz, n, phi, force, deriv, inv4piG);
         }
     } else if (path == 2) {
         BinLaunch bl = bin_launch(n);
@@ -216,10 +246,6 @@ extern "C" int agx_set_option(int key, int64_t value)
     case AGX_OPT_MP_LDS_GRID:
         if (value < 1 || value > 64) return AGX_E_SIZE;
         g_mp_lds_grid = (int)value;
-        return AGX_OK;
-    case AGX_OPT_MP_WAVES:
-        if (value != 0 && value != 1 && value != 4) return AGX_E_SIZE;
-        g_mp_waves = (int)value;
         return AGX_OK;
     default:
         return AGX_E_WHAT;

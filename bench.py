@@ -148,7 +148,7 @@ def bench_batched(dev, reps: int = 5):
     agree = float((torch.max(torch.abs(force - ref)) / torch.max(torch.abs(ref))).item())
     paths = {}
     for (pth, rows) in ens.choices:
-        k = f"{['lane', 'lds', 'binned'][pth]}/{rows}"
+        k = f"{['lane', 'lds'][pth]}/{rows}"
         paths[k] = paths.get(k, 0) + 1
     return {"n_potentials": len(pots), "n_particles": n, "work_items": int(len(ens.items)),
             "ms_batched": wall_b, "ms_batched_events": gpu_b,

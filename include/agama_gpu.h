@@ -203,16 +203,15 @@ int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xyz, int64_t 
  * d_xyz [n_rows][3]; outputs as agx_multipole_eval, rows of the concatenation.
  * d_queue is one int32 of device memory the call resets (by a kernel, not a
  * memset) and uses as the work counter; concurrent calls need separate
- * queues.  Items outside [0, n_rows) or naming no potential are skipped; a
- * path the potential does not fit (LDS table larger than
- * agx_batch_lds_table_doubles(), n_int + 1 > agx_batch_max_bins() or more
- * than 8192 rows for the binned path) falls back to AGX_BATCH_LANE.  No reference counterpart (the
- * reference launches once per potential, gpu_potential.py:757-793).
+ * queues.  Items outside [0, n_rows) or naming no potential are skipped; an
+ * AGX_BATCH_LDS item whose table is larger than
+ * agx_batch_lds_table_doubles() doubles is evaluated as AGX_BATCH_LANE.  No
+ * reference counterpart (the reference launches once per potential,
+ * gpu_potential.py:757-793).
  * ------------------------------------------------------------------------- */
-#define AGX_BATCH_MAX_ITEM 8192
+#define AGX_BATCH_MAX_ITEM 16384
 #define AGX_BATCH_LANE 0     /* one particle per lane, table reads from L2 */
 #define AGX_BATCH_LDS 1      /* table staged in LDS per item */
-#define AGX_BATCH_BINNED 2   /* item sorted by radial interval in LDS */
 typedef struct {
     int64_t start;           /* first row */
     int32_t count;           /* rows, 1..AGX_BATCH_MAX_ITEM */
@@ -225,7 +224,6 @@ int agx_batch_plan(int32_t n_pot, const int64_t *counts, const int32_t *chunk,
                    const int32_t *path, const double *cost, agx_batch_item *items,
                    int64_t capacity, int64_t *n_items);
 int64_t agx_batch_lds_table_doubles(void);
-int32_t agx_batch_max_bins(void);    /* AGX_BATCH_BINNED needs n_int + 1 <= this */
 int agx_multipole_batch_eval(const void *d_desc, int64_t desc_stride, int32_t n_pot,
                              const agx_batch_item *d_items, int64_t n_items,
                              const double *d_xyz, int64_t n_rows, double *d_phi,
@@ -261,13 +259,13 @@ int agx_leapfrog(const agx_component *d_leaves, int32_t n_leaves,
                  int32_t n_steps, void *stream);
 
 /* Tuning options (process-wide; set before launches, not thread-safe):
- *   AGX_OPT_MP_PATH      Multipole evaluation path: 0 automatic (binned for
- *                        many terms and large N, else LDS-staged when the
- *                        table fits, else per-lane), 1 one particle
+ *   AGX_OPT_MP_PATH      Multipole evaluation path: 0 automatic (LDS-staged
+ *                        when the table fits and N >= 4096, else binned for
+ *                        many terms and large N, else per-lane), 1 one particle
  *                        per lane with per-lane table reads, 2 radially binned
  *                        (wave-uniform interval, scalar table reads; needs
  *                        n_int <= 127), 3 one particle per lane with the
- *                        table staged in LDS (tables up to 64 KiB; larger
+ *                        table staged in LDS (tables up to 160 KiB; larger
  *                        ones take path 1)
  *   AGX_OPT_MP_TILE      particles per binned tile (0 = automatic, <= 8192)
  *   AGX_OPT_MP_BIN_MIN_N smallest N for which the automatic path bins
@@ -279,8 +277,6 @@ int agx_leapfrog(const agx_component *d_leaves, int32_t n_leaves,
 #define AGX_OPT_MP_TILE 2
 #define AGX_OPT_MP_BIN_MIN_N 3
 #define AGX_OPT_MP_BIN_MIN_TERMS 4
-#define AGX_OPT_MP_WAVES 5   /* LDS path register cap for 4 waves/SIMD: 0 automatic
-                                (potential and force), 1 never, 4 always */
 #define AGX_OPT_MP_LDS_GRID 6 /* LDS path grid: this many times the workgroups that
                                  fit at once (1..64, default 4) */
 #define AGX_OPT_MP_BIN_GRID 7 /* binned path grid: this many times the workgroups

@@ -5,8 +5,8 @@ An ensemble of potentials drawn like the 256-potential workload (lmax 4..12,
 and edge counts added (0, 1, one more than an item) is evaluated in one launch
 and compared with one plain launch per potential on the same rows (same
 arithmetic: 1e-13 of the scale) and, for a few potentials, with the oracle
-(tolerances of the Multipole parity tests).  Every path (lane, LDS, binned)
-and item size is forced in turn; a hipGraph replay of the batched launch
+(tolerances of the Multipole parity tests).  Both paths (lane, LDS-staged)
+and several item sizes are forced in turn; a hipGraph replay of the batched launch
 must reproduce the eager result every time.
 """
 import numpy as np
@@ -24,7 +24,7 @@ def ensemble_parts():
     from agama_gpu_amd.workloads import ensemble_coefs, ensemble_spec, hernquist_points
     spec = ensemble_spec(24, seed=3)
     counts = [max(1, s["count"] // 40) for s in spec]
-    counts[2], counts[5], counts[9] = 0, 1, 8193
+    counts[2], counts[5], counts[9] = 0, 1, 16385
     coefs = [ensemble_coefs(s) for s in spec]
     pots = [MultipolePotentialGPU(c, device=DEV) for c in coefs]
     x = hernquist_points(int(sum(counts)), 7)
@@ -35,7 +35,7 @@ def _rel(a, b):
     return (torch.max(torch.abs(a - b)) / torch.max(torch.abs(b))).item()
 
 
-@pytest.mark.parametrize("path,rows", [(0, 4096), (1, 2048), (1, 8192), (2, 8192), (2, 2048)])
+@pytest.mark.parametrize("path,rows", [(0, 4096), (0, 16384), (1, 2048), (1, 8192), (1, 16384)])
 @pytest.mark.parametrize("order", ["pot", "pf", "deriv"])
 def test_batched_equals_per_potential(ensemble_parts, path, rows, order):
     from agama_gpu_amd.batch import MultipoleEnsembleGPU

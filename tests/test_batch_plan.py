@@ -32,9 +32,9 @@ def test_plan_covers_every_row_once():
     counts = rng.integers(0, 40000, 37)
     counts[3] = 0
     counts[5] = 1
-    counts[7] = 8193
-    chunk = rng.choice([2048, 4096, 8192], 37)
-    path = rng.integers(0, 3, 37)
+    counts[7] = 16385
+    chunk = rng.choice([2048, 4096, 8192, 16384], 37)
+    path = rng.integers(0, 2, 37)
     cost = rng.uniform(100, 3000, 37)
     rc, items = _plan(counts, chunk, path, cost)
     assert rc == 0
@@ -62,7 +62,7 @@ def test_plan_rejects(bad, code):
     elif bad == "chunk_big":
         chunk = [8, _lib.AGX_BATCH_MAX_ITEM + 1]
     elif bad == "path":
-        path = [0, 3]
+        path = [0, 2]
     else:
         counts = [10, -1]
     rc, _ = _plan(counts, chunk, path, cost)
