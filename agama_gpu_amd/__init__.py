@@ -20,7 +20,7 @@ from .analytic import (AnalyticPotentialGPU, DehnenSphericalPotentialGPU, DiskAn
 from .factory import PotentialGPU
 from .batch import MultipoleEnsembleGPU
 from .integrate import LeapfrogGPU
-from .autograd import FittableMultipole, FittableShiftedMultipole, SplineCenter
+from .autograd import FittableCylSpline, FittableMultipole, FittableShiftedMultipole, SplineCenter
 
 __all__ = [
     "PotentialGPU", "PotentialBase", "MultipolePotentialGPU", "CylSplinePotentialGPU",
@@ -31,6 +31,6 @@ __all__ = [
     "LogHaloPotentialGPU", "DiskAnsatzPotentialGPU", "UniformAccelerationGPU",
     "MultipoleCoefs", "CylSplineCoefs", "read_coefs", "write_coefs", "G_AGAMA", "INV_4PIG",
     "MultipoleEnsembleGPU", "LeapfrogGPU", "FittableMultipole", "FittableShiftedMultipole",
-    "SplineCenter",
+    "SplineCenter", "FittableCylSpline",
 ]
 __version__ = "0.1.0"

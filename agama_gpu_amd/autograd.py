@@ -12,6 +12,12 @@ kernel.  The backward pass is one launch of the adjoint kernel
 wavefront-reduced scatter of dL/d(table) into the spline intervals; torch
 carries the table gradient back through the rebuild to the raw entries.
 
+``FittableCylSpline`` does the same for a CylSpline table Phi_m(R_i, z_j):
+the rebuild (z-mirroring, log-scaling, the 2-D spline node derivatives, the
+outer multipole's boundary least-squares fit) is linear algebra with
+matrices fixed at construction, and the backward pass is one launch of
+``agx_cylspline_adjoint``.
+
 ``SplineCenter`` is the differentiable counterpart of ShiftedPotentialGPU's
 trajectory: the centre at time t is linear in the knot positions, with
 weights from the same cubic spline (not-a-knot, linear extrapolation), so
@@ -27,8 +33,9 @@ import torch
 
 from . import _lib
 from .base import norm_device
-from .coefs import MultipoleCoefs
+from .coefs import CylSplineCoefs, MultipoleCoefs
 from .composite import _linear_extrapolating_spline
+from .cylspline_prep import MUL0, MUL1, build_cylspline_data, ptilde
 from .multipole_prep import build_multipole_data, pack_program
 
 F64 = torch.float64
@@ -210,6 +217,179 @@ class FittableMultipole(torch.nn.Module):
         return _MultipoleEval.apply(xyz.contiguous(), poly, scal, self)
 
 
+def _spline_slope_matrix(x: np.ndarray, bc) -> np.ndarray:
+    """D (n x n) with D @ f = the slopes at the nodes of the cubic spline
+    through values f (boundary conditions ``bc`` are homogeneous, so the map
+    is linear): the spline of each unit vector, differentiated."""
+    from scipy.interpolate import CubicSpline
+    return CubicSpline(x, np.eye(len(x)), bc_type=bc)(x, 1)
+
+
+def _outer_fit_operator(R, z, ms, lmax_fit, r0, nR, nz):
+    """The outer multipole fit of cylspline_prep.outer_asymptote as a linear map:
+    W = G @ tab[:, iRp, izp].ravel(), with the boundary points (iRp, izp) and
+    G ((lmax_fit+1)^2 x n_harm*npts) built from the least-squares
+    pseudo-inverse of each harmonic's design matrix."""
+    from scipy.linalg import lstsq
+    zsym = z[0] == 0      # never after mirroring; kept for the same point set
+    pts = []
+    for iR in range(nR - 1):
+        pts.append((R[iR], z[nz - 1], iR, nz - 1))
+        pts.append((R[iR], -z[nz - 1], iR, nz - 1) if zsym else (R[iR], z[0], iR, 0))
+    for iz in range(nz):
+        pts.append((R[nR - 1], z[iz], nR - 1, iz))
+        if zsym and iz > 0:
+            pts.append((R[nR - 1], -z[iz], nR - 1, iz))
+    Rp = np.array([p[0] for p in pts])
+    zp = np.array([p[1] for p in pts])
+    iRp = np.array([p[2] for p in pts], np.int64)
+    izp = np.array([p[3] for p in pts], np.int64)
+    rp = np.hypot(Rp, zp)
+    ct, st = zp / rp, Rp / rp
+    npts = len(pts)
+    G = np.zeros(((lmax_fit + 1) ** 2, len(ms) * npts))
+    for k, m in enumerate(ms):
+        am = abs(m)
+        if am > lmax_fit:
+            continue
+        ncol = lmax_fit - am + 1
+        mul = MUL0 if m == 0 else MUL1
+        M = np.zeros((npts, ncol))
+        for p in range(npts):
+            P = ptilde(lmax_fit, am, ct[p], st[p])
+            M[p] = P * (rp[p] / r0) ** (-(am + np.arange(ncol) + 1.0)) * mul
+        pinv = lstsq(M, np.eye(npts))[0]          # (ncol, npts)
+        for li in range(ncol):
+            l = am + li
+            G[l * (l + 1) + m, k * npts:(k + 1) * npts] = pinv[li]
+    return G, iRp, izp
+
+
+class _CylSplineEval(torch.autograd.Function):
+    """(xyz, nodes, W) -> (phi, force) with the adjoint kernel as backward."""
+
+    @staticmethod
+    def forward(ctx, xyz, nodes, W, owner):
+        desc = owner._desc_for(nodes, W)
+        n = xyz.shape[0]
+        phi = torch.empty(n, dtype=F64, device=xyz.device)
+        force = torch.empty((n, 3), dtype=F64, device=xyz.device)
+        rc = _lib.load().agx_cylspline_eval(C.byref(desc), xyz.data_ptr(), n, phi.data_ptr(),
+                                            force.data_ptr(), None, _lib.AGX_PHI | _lib.AGX_FORCE,
+                                            _lib.stream_handle(xyz.device))
+        _lib.check(rc, "agx_cylspline_eval")
+        ctx.save_for_backward(xyz, nodes, W)
+        ctx.owner = owner
+        ctx.set_materialize_grads(False)      # an unused output reaches the kernel as NULL
+        return phi, force
+
+    @staticmethod
+    def backward(ctx, gphi, gforce):
+        if gphi is None and gforce is None:
+            return None, None, None, None
+        xyz, nodes, W = ctx.saved_tensors
+        desc = ctx.owner._desc_for(nodes, W)
+        n = xyz.shape[0]
+        gx = torch.empty_like(xyz) if ctx.needs_input_grad[0] else None
+        gnodes = torch.zeros_like(nodes)
+        gW = torch.zeros_like(W)
+        gphi = None if gphi is None else gphi.contiguous()
+        gforce = None if gforce is None else gforce.contiguous()
+        rc = _lib.load().agx_cylspline_adjoint(
+            C.byref(desc), xyz.data_ptr(), n, None if gphi is None else gphi.data_ptr(),
+            None if gforce is None else gforce.data_ptr(), None if gx is None else gx.data_ptr(),
+            gnodes.data_ptr(), gW.data_ptr(), _lib.stream_handle(xyz.device))
+        _lib.check(rc, "agx_cylspline_adjoint")
+        return gx, gnodes, gW, None
+
+
+class FittableCylSpline(torch.nn.Module):
+    """CylSpline potential with its raw table Phi_m(R_i, z_j) as a torch parameter.
+
+    ``phi_table`` is (n_harm, nR, nz) in the order ``m_values`` (|m|
+    ascending, cos before sin), on the file's (R, z) grid.  forward(xyz) ->
+    (phi, force), differentiable with respect to xyz and the table.  Frozen at
+    construction, as the reference fixes them when it builds the tables
+    (gpu_potential.py:1143-1253): the grid scale Rscale = -Mtot/Phi0, the
+    outer radius r0 and whether the m=0 harmonic is log-scaled; so a gradient
+    step is exact for this model and a rebuilt CylSplinePotentialGPU of the
+    stepped table differs by the change of Rscale alone."""
+
+    def __init__(self, coefs: CylSplineCoefs, device=None):
+        super().__init__()
+        dev = norm_device(device) if device is not None else _lib.require_gpu()
+        d = build_cylspline_data(coefs)
+        self._d = d
+        self.device = dev
+        ms = [int(m) for m in d["m_arr"]]
+        self.m_values = ms
+        R = np.asarray(coefs.R_grid, np.float64)
+        z0 = np.asarray(coefs.z_grid, np.float64)
+        self._R, self._z0 = R, z0
+        self._mirror = bool(z0[0] == 0)
+        z = np.concatenate([-z0[:0:-1], z0]) if self._mirror else z0
+        nR, nz = len(R), len(z)
+        raw = np.stack([np.asarray(coefs.phi[m], np.float64) for m in ms])
+        self.phi_table = torch.nn.Parameter(torch.as_tensor(raw, device=dev))
+        G, iRp, izp = _outer_fit_operator(R, z, ms, d["lmax_outer"], d["r0_outer"], nR, nz)
+        W_np = G @ self._mirrored_np(raw)[:, iRp, izp].ravel()
+        if not np.allclose(W_np, d["W_outer"], rtol=1e-8, atol=1e-12 * np.abs(d["W_outer"]).max()):
+            raise ValueError("the outer multipole fit is degenerate for this grid")
+        self.register_buffer("G", torch.as_tensor(G, device=dev))
+        self.register_buffer("iRp", torch.as_tensor(iRp, device=dev))
+        self.register_buffer("izp", torch.as_tensor(izp, device=dev))
+        self.register_buffer("Dr", torch.as_tensor(
+            _spline_slope_matrix(d["lR_grid"], ((1, np.zeros(nR)), "natural")), device=dev))
+        self.register_buffer("Dz", torch.as_tensor(
+            _spline_slope_matrix(d["lz_grid"], "natural"), device=dev))
+        self._lR = torch.as_tensor(d["lR_grid"], device=dev)
+        self._lz = torch.as_tensor(d["lz_grid"], device=dev)
+        self._m = torch.as_tensor(d["m_arr"].astype(np.int32), device=dev)
+        self._nR, self._nz = nR, nz
+
+    def _mirrored_np(self, raw):
+        if not self._mirror:
+            return raw
+        return np.concatenate([raw[:, :, :0:-1], raw], axis=2)
+
+    # ---- differentiable table build -------------------------------------------
+    def tables(self):
+        """(nodes (n_harm, nR, nz, 4), W_outer [(lmax_outer+1)^2])."""
+        t = self.phi_table
+        full = torch.cat([t.flip(-1)[..., :-1], t], dim=-1) if self._mirror else t
+        W = self.G @ full[:, self.iRp, self.izp].reshape(-1)
+        if self._d["log_scaling"]:
+            f = torch.cat([torch.log(-full[:1]), full[1:] / full[:1]], dim=0)
+        else:
+            f = full
+        fy = f @ self.Dz.T
+        fx = self.Dr @ f
+        fxy = self.Dr @ fy
+        return torch.stack([f, fx, fy, fxy], dim=-1).contiguous(), W
+
+    def _desc_for(self, nodes: torch.Tensor, W: torch.Tensor):
+        d = self._d
+        s = _lib.agx_cylspline()
+        s.lR_grid, s.lz_grid = self._lR.data_ptr(), self._lz.data_ptr()
+        s.nodes, s.m_arr, s.W_outer = nodes.data_ptr(), self._m.data_ptr(), W.data_ptr()
+        s.nR, s.nz = self._nR, self._nz
+        s.n_harm, s.log_scaling = d["n_harm"], d["log_scaling"]
+        s.lmax_outer, s.mmax_outer = d["lmax_outer"], d["mmax_outer"]
+        s.Rscale, s.r0_outer = d["Rscale"], d["r0_outer"]
+        return s
+
+    def forward(self, xyz: torch.Tensor):
+        nodes, W = self.tables()
+        return _CylSplineEval.apply(xyz.contiguous(), nodes, W.contiguous(), self)
+
+    def to_coefs(self) -> CylSplineCoefs:
+        """The current table as a coefficient set (for write_coefs)."""
+        tab = self.phi_table.detach().cpu().numpy()
+        return CylSplineCoefs(R_grid=self._R.copy(), z_grid=self._z0.copy(),
+                              phi={m: tab[k].copy() for k, m in enumerate(self.m_values)},
+                              mmax=int(self._d["mmax"]))
+
+
 class SplineCenter(torch.nn.Module):
     """Differentiable trajectory centre c(t) from knots (T, 3) at times (T,):
     the cubic spline of ShiftedPotentialGPU (not-a-knot, linear extrapolation
@@ -235,7 +415,8 @@ class SplineCenter(torch.nn.Module):
 
 
 class FittableShiftedMultipole(torch.nn.Module):
-    """Multipole at x - c(t): gradients reach the table and the centre knots."""
+    """Expansion at x - c(t) (a FittableMultipole or FittableCylSpline):
+    gradients reach the table and the centre knots."""
 
     def __init__(self, multipole: FittableMultipole, center: SplineCenter):
         super().__init__()

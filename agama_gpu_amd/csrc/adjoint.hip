@@ -15,13 +15,6 @@
 
 namespace agx {
 
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
 // s^j a + j s^(j-1) b for j = 0..5, summed over the wavefront and added to g[0..5]
 __device__ __forceinline__ void scatter6(double *g, double s, double a, double b, int lane)
 {

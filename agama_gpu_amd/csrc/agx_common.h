@@ -144,6 +144,14 @@ __device__ __forceinline__ void for_each_point(const double *__restrict__ xyz, i
     }
 }
 
+// Sum of v over the 64 lanes of the wavefront (every lane gets the total).
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
 // Derivative order of an evaluation: potential; +gradient; +Hessian; density.
 enum { ORD_POT = 0, ORD_GRAD = 1, ORD_HESS = 2, ORD_DENS = 3 };
 

@@ -189,6 +189,20 @@ int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xyz, int64_t 
                           const double *d_gphi, const double *d_gforce, double *d_gxyz,
                           double *d_gpoly, double *d_gscal, void *stream);
 
+/* Reverse mode of agx_cylspline_eval's potential+force outputs.  Given
+ * dL/dPhi (d_gphi, [N], may be NULL) and dL/dF (d_gforce, [N][3], may be
+ * NULL) writes dL/dx into d_gxyz [N][3] (may be NULL; a Hessian-vector
+ * product) and ADDS dL/d(nodes) into d_gnodes [n_harm][nR][nz][4] and
+ * dL/d(W_outer) into d_gW [(lmax_outer+1)^2]; the caller zeroes those first.
+ * Node gradients take one FP64 atomic add per touched node quantity, outer
+ * coefficients one per wavefront, so their last bits depend on the order of
+ * the atomics.  No reference counterpart (the reference has no derivatives
+ * with respect to positions or tables); it serves torch.autograd for fitting
+ * disc potentials to tracer forces. */
+int agx_cylspline_adjoint(const agx_cylspline *cs, const double *d_xyz, int64_t n,
+                          const double *d_gphi, const double *d_gforce, double *d_gxyz,
+                          double *d_gnodes, double *d_gW, void *stream);
+
 /* ---------------------------------------------------------------------------
  * Ragged ensembles: many independent Multipole potentials, each with its own
  * particle range of the concatenated arrays, in one persistent launch.

@@ -71,3 +71,42 @@ def test_spline_center_weights():
         g = torch.autograd.grad(sc(t)[1], sc.knots)[0].numpy()
         assert np.allclose(g[:, 1], w, rtol=0, atol=1e-15)
         assert np.allclose(g[:, [0, 2]], 0.0)
+
+
+def test_cylspline_tables_match_numpy_build():
+    """The CylSpline rebuild (mirroring, log-scaling, spline slopes by matrix,
+    outer fit by pseudo-inverse) reproduces cylspline_prep's node data and W."""
+    from agama_gpu_amd.autograd import FittableCylSpline
+    from agama_gpu_amd.coefs import read_coefs
+    from agama_gpu_amd.cylspline_prep import build_cylspline_data
+    c = read_coefs(FIXTURES / "600.bar.none_8.coef_cylsp_DR")
+    d = build_cylspline_data(c)
+    m = FittableCylSpline(c, device="cpu")
+    with torch.no_grad():
+        nodes, W = m.tables()
+    nodes, ref = nodes.numpy(), d["node_arr"]
+    assert nodes.shape == ref.shape
+    # per (harmonic, quantity) scale
+    scale = np.maximum(np.abs(ref).max(axis=(1, 2)), 1e-300)[:, None, None, :]
+    assert np.max(np.abs(nodes - ref) / scale) < 1e-11
+    Wref = d["W_outer"]
+    assert np.max(np.abs(W.numpy() - Wref)) <= 1e-10 * np.abs(Wref).max()
+    # round trip through the coefficient set
+    back = m.to_coefs()
+    for k, mv in enumerate(m.m_values):
+        assert np.array_equal(back.phi[mv], np.asarray(c.phi[mv]))
+
+
+def test_cylspline_table_gradient_is_local_in_m():
+    """d(nodes of harmonic k)/d(table) touches harmonic k and, through the
+    log-scaling ratio, the m=0 table only."""
+    from agama_gpu_amd.autograd import FittableCylSpline
+    from agama_gpu_amd.coefs import read_coefs
+    c = read_coefs(FIXTURES / "600.bar.none_8.coef_cylsp_DR")
+    m = FittableCylSpline(c, device="cpu")
+    nodes, _ = m.tables()
+    k = 3
+    g = torch.autograd.grad(nodes[k].sum(), m.phi_table)[0].numpy()
+    assert np.isfinite(g).all()
+    touched = set(np.nonzero(np.abs(g).max(axis=(1, 2)))[0].tolist())
+    assert touched <= {0, k} and k in touched
