@@ -18,14 +18,9 @@ namespace agx {
 // s^j a + j s^(j-1) b for j = 0..5, summed over the wavefront and added to g[0..5]
 __device__ __forceinline__ void scatter6(double *g, double s, double a, double b, int lane)
 {
-    double sj = 1.0, sjm = 0.0;   // s^j, j s^(j-1)
-#pragma unroll
-    for (int j = 0; j < 6; ++j) {
-        const double v = wave_sum(fma(sj, a, sjm * b));
-        if (lane == 0) atomicAdd(g + j, v);
-        sjm = (double)(j + 1) * sj;
-        sj *= s;
-    }
+    const double s2 = s * s, s3 = s2 * s, s4 = s2 * s2;
+    wave_sum6_add(g, a, fma(s, a, b), fma(s2, a, 2.0 * s * b), fma(s3, a, 3.0 * s2 * b),
+                  fma(s4, a, 4.0 * s3 * b), fma(s4 * s, a, 5.0 * s4 * b), lane);
 }
 
 __device__ __forceinline__ void adjoint_interior(const MPK &P, int k, bool valid, double x, double y,

@@ -144,12 +144,84 @@ __device__ __forceinline__ void for_each_point(const double *__restrict__ xyz, i
     }
 }
 
+// ---- cross-lane sums without LDS -------------------------------------------
+// DPP move of both 32-bit halves of a double (all rows and banks enabled).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false);
+}
+
+// gfx950 lane swaps of a pair of doubles, both halves:
+//   ROW16 = false (v_permlane32_swap): lanes 32-63 of a trade places with lanes 0-31 of b;
+//   ROW16 = true  (v_permlane16_swap): rows 1 and 3 of a (16 lanes each) trade places
+//                                      with rows 0 and 2 of b.
+template <bool ROW16>
+__device__ __forceinline__ void lane_swap(double &a, double &b)
+{
+    const unsigned long long ua = __builtin_bit_cast(unsigned long long, a);
+    const unsigned long long ub = __builtin_bit_cast(unsigned long long, b);
+    const unsigned alo = (unsigned)ua, ahi = (unsigned)(ua >> 32);
+    const unsigned blo = (unsigned)ub, bhi = (unsigned)(ub >> 32);
+    unsigned r0, r1, r2, r3;
+    if constexpr (ROW16) {
+        const auto lo = __builtin_amdgcn_permlane16_swap(alo, blo, false, false);
+        const auto hi = __builtin_amdgcn_permlane16_swap(ahi, bhi, false, false);
+        r0 = lo[0]; r1 = lo[1]; r2 = hi[0]; r3 = hi[1];
+    } else {
+        const auto lo = __builtin_amdgcn_permlane32_swap(alo, blo, false, false);
+        const auto hi = __builtin_amdgcn_permlane32_swap(ahi, bhi, false, false);
+        r0 = lo[0]; r1 = lo[1]; r2 = hi[0]; r3 = hi[1];
+    }
+    a = __builtin_bit_cast(double, ((unsigned long long)r2 << 32) | r0);
+    b = __builtin_bit_cast(double, ((unsigned long long)r3 << 32) | r1);
+}
+
+// Sum over the 16 lanes of each row (every lane of the row gets its row's sum).
+__device__ __forceinline__ double row_sum(double v)
+{
+    v += dpp_f64<0x128>(v);   // row_ror:8
+    v += dpp_f64<0x124>(v);   // row_ror:4
+    v += dpp_f64<0x4E>(v);    // quad_perm [2,3,0,1]
+    v += dpp_f64<0xB1>(v);    // quad_perm [1,0,3,2]
+    return v;
+}
+
 // Sum of v over the 64 lanes of the wavefront (every lane gets the total).
+// Every lane must be active.
 __device__ __forceinline__ double wave_sum(double v)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    double a = v, b = v;
+    lane_swap<false>(a, b);
+    v = a + b;                // lane i: v_i + v_(i^32)
+    a = v; b = v;
+    lane_swap<true>(a, b);
+    return row_sum(a + b);    // + the paired row, then the row
+}
+
+// Wavefront sums of v0..v5 added to g[0..5] by FP64 atomics (every lane must
+// be active).  The swaps transpose as they add: after the 32-lane swaps the
+// low half of a register holds one quantity's pair sums and the high half
+// another's; after the 16-lane swaps each row holds a different quantity, so
+// six sums cost five swaps and two row reductions, and the adds are two
+// atomic instructions (4 and 2 lanes).
+__device__ __forceinline__ void wave_sum6_add(double *g, double v0, double v1, double v2,
+                                              double v3, double v4, double v5, int lane)
+{
+    lane_swap<false>(v0, v1);
+    lane_swap<false>(v2, v3);
+    lane_swap<false>(v4, v5);
+    double a = v0 + v1;      // lanes 0-31: quantity 0, lanes 32-63: quantity 1
+    double b = v2 + v3;      // 2 | 3
+    double c = v4 + v5;      // 4 | 5
+    lane_swap<true>(a, b);
+    double d = row_sum(a + b);   // rows: 0, 2, 1, 3
+    double c2 = c;
+    lane_swap<true>(c, c2);
+    double e = row_sum(c + c2);  // rows: 4, 4, 5, 5
+    const int row = lane >> 4;
+    if ((lane & 15) == 0) atomicAdd(g + (((row & 1) << 1) | (row >> 1)), d);
+    if ((lane & 31) == 0) atomicAdd(g + 4 + (row >> 1), e);
 }
 
 // Derivative order of an evaluation: potential; +gradient; +Hessian; density.
