@@ -34,6 +34,7 @@ class agx_multipole(C.Structure):
         ("invPhi0", C.c_double),
         ("inner_s", C.c_double), ("inner_U", C.c_double), ("inner_W", C.c_double),
         ("outer_s", C.c_double), ("outer_U", C.c_double), ("outer_W", C.c_double),
+        ("law", C.c_void_p),
     ]
 
 
@@ -98,7 +99,8 @@ _SIGNATURES = {
                                         C.c_int64, C.c_void_p, C.c_void_p]),
     "agx_multipole_adjoint": (C.c_int, [C.POINTER(agx_multipole), C.c_void_p, C.c_int64,
                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                        C.c_void_p, C.c_void_p]),
+                                        C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    "agx_multipole_adjoint_work_size": (C.c_int64, [C.c_int64]),
     "agx_cylspline_eval": (C.c_int, [C.POINTER(agx_cylspline), C.c_void_p, C.c_int64,
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "agx_cylspline_density": (C.c_int, [C.POINTER(agx_cylspline), C.c_void_p, C.c_int64,

@@ -1,35 +1,152 @@
 // Adjoint (reverse-mode) kernel of the Multipole potential+force evaluation.
 //
 // Given upstream gradients w_phi = dL/dPhi_i and w_F = dL/dF_i (F = -grad Phi)
-// at N points, one launch produces
+// at N points, one call produces
 //   dL/dx_i   = w_phi grad Phi - H w_F            (Hessian-vector product)
 //   dL/dpoly  [n_int][n_lm][6]  (Horner coefficients of every term, interval)
 //   dL/dscal  [invPhi0, inner U, inner W, outer U, outer W]
-// The table gradient is a scatter into the interval a particle falls in: the
-// particles are sorted by interval per tile (binned_tiles), so a wavefront's
-// 64 lanes share the interval; each coefficient's contribution is summed
-// across the wavefront first and added to memory with one FP64 atomic per
-// wavefront.  No reference counterpart: the reference has no derivatives with
-// respect to its tables.
+// The table gradient is a scatter into the interval a particle falls in.  The
+// particles are first sorted by interval over the whole array (a counting
+// sort into the caller's workspace: bin, scan, scatter), so that every
+// wavefront of the adjoint kernel takes 64 particles of one interval (only a
+// bin's last chunk is partial).  Each term's six coefficient sums are reduced
+// across the wavefront by lane swaps and added to a wavefront-private LDS
+// copy of that interval's table slab; the slab goes to memory (FP64 atomics)
+// only when the wavefront moves to another interval.  Wavefronts take runs of
+// consecutive chunks from a queue counter.  No reference counterpart: the
+// reference has no derivatives with respect to its tables.
 #include "multipole_dev.h"
 
 namespace agx {
 
-// s^j a + j s^(j-1) b for j = 0..5, summed over the wavefront and added to g[0..5]
+constexpr int kAdjBlock = 256;
+constexpr int64_t kAdjHeader = 2048;     // bytes of counters at the start of the workspace
+// LDS slabs are used while 4 of them (one per wavefront) fit in this many bytes
+constexpr int kAdjSlabLds = 53 * 1024;
+
+// The workspace: counters, each particle's bin, the sorted particle indices.
+struct AdjWork {
+    int *hist;      // [kMaxBins] particles per bin
+    int *off;       // [kMaxBins + 1] first slot of each bin; bins padded to 64 slots
+    int *cursor;    // [kMaxBins] next free slot of each bin while scattering
+    int *queue;     // [1] next run of chunks
+    uint8_t *bin;   // [n]
+    int *idx;       // [n + 64 kMaxBins] particle of each slot
+};
+
+inline int64_t adj_work_size(int64_t n)
+{
+    return kAdjHeader + ((n + 255) & ~(int64_t)255) + 4 * (n + 64 * (int64_t)kMaxBins);
+}
+
+inline AdjWork adj_work(void *base, int64_t n)
+{
+    AdjWork w;
+    int *h = (int *)base;
+    w.hist = h;
+    w.off = h + kMaxBins;
+    w.cursor = h + 2 * kMaxBins + 1;
+    w.queue = h + 3 * kMaxBins + 1;
+    w.bin = (uint8_t *)base + kAdjHeader;
+    w.idx = (int *)((uint8_t *)base + kAdjHeader + ((n + 255) & ~(int64_t)255));
+    return w;
+}
+
+__device__ __forceinline__ int bin_of(const MPK &P, double x, double y, double z)
+{
+    const Where w = classify(P, x, y, z);
+    return (w.kind == 2) ? w.k + 1 : 0;
+}
+
+__global__ __launch_bounds__(kAdjBlock) void adj_init_kernel(AdjWork W)
+{
+    for (int b = threadIdx.x; b < kMaxBins; b += kAdjBlock) W.hist[b] = 0;
+    if (threadIdx.x == 0) W.queue[0] = 0;
+}
+
+// Bin of every particle, and the histogram (LDS per workgroup, then global).
+__global__ __launch_bounds__(kAdjBlock) void adj_count_kernel(MPK P, const double *__restrict__ xyz,
+                                                             int64_t n, AdjWork W)
+{
+    __shared__ int cnt[kMaxBins];
+    const int nb = P.n_int + 1;
+    for (int b = threadIdx.x; b < nb; b += kAdjBlock) cnt[b] = 0;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * kAdjBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kAdjBlock) {
+        const int b = bin_of(P, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+        W.bin[i] = (uint8_t)b;
+        atomicAdd(&cnt[b], 1);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += kAdjBlock)
+        if (cnt[b]) atomicAdd(&W.hist[b], cnt[b]);
+}
+
+// Exclusive scan of the 64-padded bin sizes (one thread: at most 128 bins).
+__global__ void adj_scan_kernel(AdjWork W, int nb)
+{
+    if (threadIdx.x != 0) return;
+    int o = 0;
+    for (int b = 0; b < nb; ++b) {
+        W.off[b] = o;
+        W.cursor[b] = o;
+        o += (W.hist[b] + 63) & ~63;
+    }
+    W.off[nb] = o;
+}
+
+// Particle indices into their bins' slots: each workgroup reserves one range
+// per bin (one global atomic), then places its particles with LDS counters.
+// Same particle-to-workgroup mapping as adj_count_kernel.
+__global__ __launch_bounds__(kAdjBlock) void adj_scatter_kernel(int64_t n, int nb, AdjWork W)
+{
+    __shared__ int cnt[kMaxBins], base[kMaxBins];
+    for (int b = threadIdx.x; b < nb; b += kAdjBlock) cnt[b] = 0;
+    __syncthreads();
+    const int64_t i0 = (int64_t)blockIdx.x * kAdjBlock + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * kAdjBlock;
+    for (int64_t i = i0; i < n; i += stride) atomicAdd(&cnt[W.bin[i]], 1);
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += kAdjBlock) {
+        base[b] = cnt[b] ? atomicAdd(&W.cursor[b], cnt[b]) : 0;
+        cnt[b] = 0;
+    }
+    __syncthreads();
+    for (int64_t i = i0; i < n; i += stride) {
+        const int b = W.bin[i];
+        W.idx[base[b] + atomicAdd(&cnt[b], 1)] = (int)i;
+    }
+}
+
+// s^j a + j s^(j-1) b for j = 0..5, summed over the wavefront and added to
+// g[0..5]: a wavefront-private LDS slab (SLAB) or the table gradient in memory.
+template <bool SLAB>
 __device__ __forceinline__ void scatter6(double *g, double s, double a, double b, int lane)
 {
     const double s2 = s * s, s3 = s2 * s, s4 = s2 * s2;
-    wave_sum6_add(g, a, fma(s, a, b), fma(s2, a, 2.0 * s * b), fma(s3, a, 3.0 * s2 * b),
-                  fma(s4, a, 4.0 * s3 * b), fma(s4 * s, a, 5.0 * s4 * b), lane);
+    const double v1 = fma(s, a, b), v2 = fma(s2, a, 2.0 * s * b), v3 = fma(s3, a, 3.0 * s2 * b);
+    const double v4 = fma(s4, a, 4.0 * s3 * b), v5 = fma(s4 * s, a, 5.0 * s4 * b);
+    if constexpr (SLAB) {
+        double d, e;
+        wave_sum6(a, v1, v2, v3, v4, v5, d, e);
+        const int row = lane >> 4;
+        if ((lane & 15) == 0) g[((row & 1) << 1) | (row >> 1)] += d;
+        if ((lane & 31) == 0) g[4 + (row >> 1)] += e;
+    } else {
+        wave_sum6_add(g, a, v1, v2, v3, v4, v5, lane);
+    }
 }
 
+// One chunk of interval k.  gslab: where this interval's coefficient sums go
+// (LDS slab or memory); the invPhi0 gradient is accumulated per lane in ginv.
+template <bool SLAB>
 __device__ __forceinline__ void adjoint_interior(const MPK &P, int k, bool valid, double x, double y,
                                                  double z, double wphi, const double wf[3],
-                                                 double *gx, double *gpoly, double *gscal,
+                                                 double *gx, double *gslab, double &ginv_acc,
                                                  int lane)
 {
     const double *slab = P.poly + (int64_t)k * P.n_lm * 6;
-    double *gslab = gpoly + (int64_t)k * P.n_lm * 6;
     if (!valid) {   // a finite point inside interval k; its seeds are zero
         x = exp(fma((double)k + 0.5, P.dlogr, P.logr_min)); y = 0.0; z = 0.0;
         wphi = 0.0;
@@ -115,7 +232,7 @@ __device__ __forceinline__ void adjoint_interior(const MPK &P, int k, bool valid
                 Sv = fma(C, QT, Sv); Sr = fma(Cd, QT, Sr); St = fma(C, dQ1 * T, St); Sx = fma(C, X, Sx);
                 const double alpha = lam * (wphi * QT + wPt * dQ1 * T + wPf * X) + wPr * mu * QT;
                 const double beta = wPr * lam * idl * QT;
-                scatter6(gslab + off, s, alpha, beta, lane);
+                scatter6<SLAB>(gslab + off, s, alpha, beta, lane);
             }
         }
     }
@@ -131,14 +248,15 @@ __device__ __forceinline__ void adjoint_interior(const MPK &P, int k, bool valid
         gc0 = wphi;
         gc0d = wPr * idl;
     }
-    scatter6(gslab, s, gc0, gc0d, lane);
-    const double gi = wave_sum(ginv);
-    if (lane == 0 && P.log_scaling) atomicAdd(gscal + 0, gi);
+    scatter6<SLAB>(gslab, s, gc0, gc0d, lane);
+    ginv_acc += ginv;
 }
 
+// One chunk outside the grid; the law gradients are accumulated per lane in
+// acc = [inner U, inner W, outer U, outer W].
 __device__ __forceinline__ void adjoint_extrap(const MPK &P, bool valid, double x, double y,
                                                double z, double wphi, const double wf[3],
-                                               double *gx, double *gscal, int lane)
+                                               double *gx, double acc[4])
 {
     double gU_in = 0.0, gW_in = 0.0, gU_out = 0.0, gW_out = 0.0;
     if (valid) {
@@ -168,59 +286,131 @@ __device__ __forceinline__ void adjoint_extrap(const MPK &P, bool valid, double 
             (void)U; (void)W;
         }
     }
-    gU_in = wave_sum(gU_in); gW_in = wave_sum(gW_in);
-    gU_out = wave_sum(gU_out); gW_out = wave_sum(gW_out);
-    if (lane == 0) {
-        atomicAdd(gscal + 1, gU_in); atomicAdd(gscal + 2, gW_in);
-        atomicAdd(gscal + 3, gU_out); atomicAdd(gscal + 4, gW_out);
-    }
+    acc[0] += gU_in; acc[1] += gW_in; acc[2] += gU_out; acc[3] += gW_out;
 }
 
-__global__ __launch_bounds__(kBinThreads) void mp_adjoint_kernel(
-    MPK P, const double *__restrict__ xyz, int64_t n, int64_t tile, const double *__restrict__ gphi,
-    const double *__restrict__ gforce, double *__restrict__ gxyz, double *__restrict__ gpoly,
-    double *__restrict__ gscal)
+// Adds a wavefront's LDS slab to memory and clears it.
+__device__ __forceinline__ void flush_slab(double *slab, double *g, int cnt, int lane)
 {
+    __builtin_amdgcn_wave_barrier();
+    for (int q = lane; q < cnt; q += 64) {
+        const double v = slab[q];
+        if (v != 0.0) atomicAdd(g + q, v);
+        slab[q] = 0.0;
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Runs of run_len consecutive 64-slot chunks of the sorted order, taken from
+// the queue by whole wavefronts until none is left.
+template <bool SLAB>
+__global__ __launch_bounds__(kAdjBlock) void mp_adjoint_kernel(
+    MPK P, const double *__restrict__ xyz, const double *__restrict__ gphi,
+    const double *__restrict__ gforce, double *__restrict__ gxyz, double *__restrict__ gpoly,
+    double *__restrict__ gscal, AdjWork W, int run_len)
+{
+    P = with_law(P);
+    extern __shared__ double adj_lds[];
     const int lane = threadIdx.x & 63;
-    binned_tiles(P, xyz, n, tile, [&](int b, bool valid, int64_t i) {
-        double x = 1.0, y = 0.0, z = 0.0, wphi = 0.0, wf[3] = {0.0, 0.0, 0.0};
-        if (valid) {
-            x = xyz[3 * i]; y = xyz[3 * i + 1]; z = xyz[3 * i + 2];
-            if (gphi) wphi = gphi[i];
-            if (gforce) { wf[0] = gforce[3 * i]; wf[1] = gforce[3 * i + 1]; wf[2] = gforce[3 * i + 2]; }
+    const int slab_n = P.n_lm * 6;
+    double *slab = adj_lds + (SLAB ? (threadIdx.x >> 6) * slab_n : 0);
+    if (SLAB)
+        for (int q = lane; q < slab_n; q += 64) slab[q] = 0.0;
+    const int nb = P.n_int + 1;
+    const int total = cldi(W.off, nb) >> 6;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};   // invPhi0, inner U, W, outer U, W
+    int cur = -1;                                // interval + 1 held by the slab
+    for (;;) {
+        int run = 0;
+        if (lane == 0) run = atomicAdd(W.queue, 1);
+        run = __builtin_amdgcn_readfirstlane(run);
+        const int c0 = run * run_len;
+        if (c0 >= total) break;
+        const int c1 = min(c0 + run_len, total);
+        // bin of chunk c0: the last b with off[b] <= 64 c0 (off[0] = 0 <= 64 c0 < off[nb])
+        int b = 0, hi = nb;
+        while (hi - b > 1) {
+            const int mid = (b + hi) >> 1;
+            if (cldi(W.off, mid) <= c0 * 64) b = mid; else hi = mid;
         }
-        double gx[3] = {0.0, 0.0, 0.0};
-        if (b > 0)
-            adjoint_interior(P, b - 1, valid, x, y, z, wphi, wf, gx, gpoly, gscal, lane);
-        else
-            adjoint_extrap(P, valid, x, y, z, wphi, wf, gx, gscal, lane);
-        if (valid && gxyz) { gxyz[3 * i] = gx[0]; gxyz[3 * i + 1] = gx[1]; gxyz[3 * i + 2] = gx[2]; }
-    });
+        for (int c = c0; c < c1; ++c) {
+            while (cldi(W.off, b + 1) <= c * 64) ++b;
+            if (SLAB && b != cur) {
+                if (cur > 0) flush_slab(slab, gpoly + (int64_t)(cur - 1) * slab_n, slab_n, lane);
+                cur = b;
+            }
+            const int sl = c * 64 + lane;
+            const bool valid = sl < cldi(W.off, b) + cldi(W.hist, b);
+            double x = 1.0, y = 0.0, z = 0.0, wphi = 0.0, wf[3] = {0.0, 0.0, 0.0};
+            int64_t i = 0;
+            if (valid) {
+                i = W.idx[sl];
+                x = xyz[3 * i]; y = xyz[3 * i + 1]; z = xyz[3 * i + 2];
+                if (gphi) wphi = gphi[i];
+                if (gforce) { wf[0] = gforce[3 * i]; wf[1] = gforce[3 * i + 1]; wf[2] = gforce[3 * i + 2]; }
+            }
+            double gx[3] = {0.0, 0.0, 0.0};
+            if (b > 0)
+                adjoint_interior<SLAB>(P, b - 1, valid, x, y, z, wphi, wf, gx,
+                                       SLAB ? slab : gpoly + (int64_t)(b - 1) * slab_n, acc[0], lane);
+            else
+                adjoint_extrap(P, valid, x, y, z, wphi, wf, gx, acc + 1);
+            if (valid && gxyz) { gxyz[3 * i] = gx[0]; gxyz[3 * i + 1] = gx[1]; gxyz[3 * i + 2] = gx[2]; }
+        }
+    }
+    if (SLAB && cur > 0) flush_slab(slab, gpoly + (int64_t)(cur - 1) * slab_n, slab_n, lane);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        const double v = wave_sum(acc[q]);
+        if (lane == 0 && v != 0.0) atomicAdd(gscal + q, v);
+    }
 }
 
 }  // namespace agx
 
 using namespace agx;
 
+extern "C" int64_t agx_multipole_adjoint_work_size(int64_t n)
+{
+    return n < 0 ? 0 : adj_work_size(n);
+}
+
 extern "C" int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xyz, int64_t n,
                                      const double *d_gphi, const double *d_gforce, double *d_gxyz,
-                                     double *d_gpoly, double *d_gscal, void *stream)
+                                     double *d_gpoly, double *d_gscal, void *d_work,
+                                     int64_t work_bytes, void *stream)
 {
     if (!mp) return AGX_E_NULL;
-    if (n < 0) return AGX_E_SIZE;
+    if (n < 0 || n > (int64_t)0x7FFFFFFF - 64 * kMaxBins) return AGX_E_SIZE;
     if (!mp->poly_km || !mp->program || !d_gpoly || !d_gscal) return AGX_E_NULL;
-    if (mp->n_int + 1 > kMaxBins) return AGX_E_SIZE;
+    if (mp->n_int < 1 || mp->n_int + 1 > kMaxBins || mp->n_lm < 1) return AGX_E_SIZE;
     if (mp->lmax > 32) return AGX_E_LMAX;
     if (n == 0) return AGX_OK;
-    if (!d_xyz || (!d_gphi && !d_gforce)) return AGX_E_NULL;
+    if (!d_xyz || (!d_gphi && !d_gforce) || !d_work) return AGX_E_NULL;
+    if (work_bytes < adj_work_size(n)) return AGX_E_WORKSPACE;
     MPK P = make_args(mp);
-    const int64_t max_wg = 256 * 3;
-    int64_t tile = (n + max_wg - 1) / max_wg;
-    if (tile < 1024) tile = 1024;
-    if (tile > kMaxTile) tile = kMaxTile;
-    int64_t n_tiles = (n + tile - 1) / tile;
-    const unsigned grid = (unsigned)(n_tiles < max_wg ? n_tiles : max_wg);
-    hipLaunchKernelGGL(mp_adjoint_kernel, dim3(grid), dim3(kBinThreads), 0, (hipStream_t)stream, P,
-                       d_xyz, n, tile, d_gphi, d_gforce, d_gxyz, d_gpoly, d_gscal);
+    const AdjWork W = adj_work(d_work, n);
+    const int nb = mp->n_int + 1;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t n_blk = (n + kAdjBlock - 1) / kAdjBlock;
+    const unsigned sort_grid = (unsigned)(n_blk < 1024 ? n_blk : 1024);
+    hipLaunchKernelGGL(adj_init_kernel, dim3(1), dim3(kAdjBlock), 0, st, W);
+    hipLaunchKernelGGL(adj_count_kernel, dim3(sort_grid), dim3(kAdjBlock), 0, st, P, d_xyz, n, W);
+    hipLaunchKernelGGL(adj_scan_kernel, dim3(1), dim3(64), 0, st, W, nb);
+    hipLaunchKernelGGL(adj_scatter_kernel, dim3(sort_grid), dim3(kAdjBlock), 0, st, n, nb, W);
+    // at most this many chunks; runs of run_len chunks, about two per resident wavefront
+    const int64_t chunks = (n + 63) / 64 + nb;
+    const int64_t waves = 256LL * 12;
+    int run_len = (int)((chunks + 2 * waves - 1) / (2 * waves));
+    if (run_len < 1) run_len = 1;
+    const int64_t wg_need = (chunks + run_len - 1) / run_len / (kAdjBlock / 64) + 1;
+    const unsigned grid = (unsigned)(wg_need < 1024 ? wg_need : 1024);
+    const size_t slab_bytes = (size_t)mp->n_lm * 6 * sizeof(double) * (kAdjBlock / 64);
+    if (slab_bytes <= (size_t)kAdjSlabLds)
+        hipLaunchKernelGGL(mp_adjoint_kernel<true>, dim3(grid), dim3(kAdjBlock), slab_bytes, st, P,
+                           d_xyz, d_gphi, d_gforce, d_gxyz, d_gpoly, d_gscal, W, run_len);
+    else
+        hipLaunchKernelGGL(mp_adjoint_kernel<false>, dim3(grid), dim3(kAdjBlock), 0, st, P, d_xyz,
+                           d_gphi, d_gforce, d_gxyz, d_gpoly, d_gscal, W, run_len);
     return hipGetLastError() == hipSuccess ? AGX_OK : AGX_E_LAUNCH;
 }

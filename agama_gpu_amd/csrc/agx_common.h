@@ -205,8 +205,15 @@ __device__ __forceinline__ double wave_sum(double v)
 // another's; after the 16-lane swaps each row holds a different quantity, so
 // six sums cost five swaps and two row reductions, and the adds are two
 // atomic instructions (4 and 2 lanes).
-__device__ __forceinline__ void wave_sum6_add(double *g, double v0, double v1, double v2,
-                                              double v3, double v4, double v5, int lane)
+// Transposed wavefront sums of v0..v5: on return, row r (lanes 16r..16r+15) of d
+// holds the total of quantity ((r & 1) << 1) | (r >> 1), i.e. rows 0..3 hold
+// quantities 0, 2, 1, 3, and rows 0-1 / 2-3 of e hold quantities 4 / 5.
+// Every lane must be active.  The swaps transpose as they add: after the
+// 32-lane swaps the low half of a register holds one quantity's pair sums and
+// the high half another's; after the 16-lane swaps each row holds a different
+// quantity, so six sums cost five swaps and two row reductions.
+__device__ __forceinline__ void wave_sum6(double v0, double v1, double v2, double v3, double v4,
+                                          double v5, double &d, double &e)
 {
     lane_swap<false>(v0, v1);
     lane_swap<false>(v2, v3);
@@ -215,10 +222,18 @@ __device__ __forceinline__ void wave_sum6_add(double *g, double v0, double v1, d
     double b = v2 + v3;      // 2 | 3
     double c = v4 + v5;      // 4 | 5
     lane_swap<true>(a, b);
-    double d = row_sum(a + b);   // rows: 0, 2, 1, 3
+    d = row_sum(a + b);
     double c2 = c;
     lane_swap<true>(c, c2);
-    double e = row_sum(c + c2);  // rows: 4, 4, 5, 5
+    e = row_sum(c + c2);
+}
+
+// wave_sum6, added to g[0..5] by FP64 atomics (two atomic instructions, 4 and 2 lanes).
+__device__ __forceinline__ void wave_sum6_add(double *g, double v0, double v1, double v2,
+                                              double v3, double v4, double v5, int lane)
+{
+    double d, e;
+    wave_sum6(v0, v1, v2, v3, v4, v5, d, e);
     const int row = lane >> 4;
     if ((lane & 15) == 0) atomicAdd(g + (((row & 1) << 1) | (row >> 1)), d);
     if ((lane & 31) == 0) atomicAdd(g + 4 + (row >> 1), e);

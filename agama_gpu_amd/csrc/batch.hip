@@ -80,7 +80,7 @@ __global__ __launch_bounds__(kBatchBlock) void mp_batch_kernel(
         const int count = cldi(iw, 2), pot = cldi(iw, 3), path = cldi(iw, 4);
         // an item outside the arrays or naming no potential is skipped
         if (start < 0 || count < 1 || start > n_rows - count || pot < 0 || pot >= n_pot) continue;
-        const MPK P = load_desc<MPK>(descs + (int64_t)pot * stride);
+        const MPK P = with_law(load_desc<MPK>(descs + (int64_t)pot * stride));
         const int tab_len = P.n_int * P.n_lm * 6;
         if (path == AGX_BATCH_LDS && tab_len <= kBatchTableDoubles) {
             const dvec2 *src = (const dvec2 *)P.poly;

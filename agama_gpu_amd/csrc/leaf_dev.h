@@ -22,7 +22,7 @@ __device__ __forceinline__ void add_leaf(int type, const double *p, const void *
     const double x = sg * (x0 - center[0]), y = sg * (y0 - center[1]), z = sg * (z0 - center[2]);
     Eval3 e;
     if (type == AGX_T_MULTIPOLE) {
-        const MPK mp = load_desc<MPK>(table);
+        const MPK mp = with_law(load_desc<MPK>(table));
         PointOut o{};
         Where w = classify(mp, x, y, z);
         if (w.kind == 2)

@@ -10,6 +10,7 @@ __global__ __launch_bounds__(kBlock) void mp_lane_kernel(MPK P, const double *__
                                                          double *__restrict__ force,
                                                          double *__restrict__ deriv, double inv4piG)
 {
+    P = with_law(P);
     for_each_point<kBlock>(xyz, n, [&](int64_t i, double x, double y, double z) {
         PointOut o{};
         Where w = classify(P, x, y, z);
@@ -27,6 +28,7 @@ __global__ __launch_bounds__(kBinThreads) void mp_binned_kernel(
     MPK P, const double *__restrict__ xyz, int64_t n, int64_t tile, double *__restrict__ phi,
     double *__restrict__ force, double *__restrict__ deriv, double inv4piG)
 {
+    P = with_law(P);
     binned_tiles(P, xyz, n, tile, [&](int b, bool valid, int64_t i) {
         if (!valid) return;
         const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
@@ -55,6 +57,7 @@ __global__ __launch_bounds__(BLOCK) void mp_lds_kernel(MPK P, const double *__re
                                                         double *__restrict__ force,
                                                         double *__restrict__ deriv, double inv4piG)
 {
+    P = with_law(P);
     extern __shared__ __align__(16) double lds_tab[];
     // Positions are read one grid-stride iteration ahead (the first before
     // the table staging), so the HBM latency of the next particle overlaps

@@ -37,7 +37,20 @@ struct MPK {
     double logr_min, dlogr, inv_dlogr, invPhi0;
     double inner_s, inner_U, inner_W, outer_s, outer_U, outer_W;
     double inv_rmin, inv_rmax;       // 1 / r at the first and last node
+    const double *law;               // NULL or device [invPhi0, inner U, W, outer U, W]
 };
+
+// The table with its five law scalars taken from device memory when it
+// carries them there (agx_multipole.law); called once per kernel or descriptor.
+__device__ __forceinline__ MPK with_law(MPK P)
+{
+    if (P.law) {
+        P.invPhi0 = cld(P.law, 0);
+        P.inner_U = cld(P.law, 1); P.inner_W = cld(P.law, 2);
+        P.outer_U = cld(P.law, 3); P.outer_W = cld(P.law, 4);
+    }
+    return P;
+}
 
 
 // l=0 extrapolation law and its first two ln r derivatives.
@@ -503,6 +516,7 @@ inline MPK make_args(const agx_multipole *mp)
     P.outer_s = mp->outer_s; P.outer_U = mp->outer_U; P.outer_W = mp->outer_W;
     P.inv_rmin = std::exp(-mp->logr_min);
     P.inv_rmax = std::exp(-(mp->logr_min + (double)mp->n_int * mp->dlogr));
+    P.law = mp->law;
     return P;
 }
 

@@ -74,6 +74,11 @@ typedef struct agx_multipole {
     double logr_min, dlogr, inv_dlogr, invPhi0;
     double inner_s, inner_U, inner_W;
     double outer_s, outer_U, outer_W;
+    /* NULL, or device memory holding [invPhi0, inner_U, inner_W, outer_U,
+     * outer_W], read by the kernels in place of the five fields above: lets a
+     * caller whose table is built on the device (torch.autograd) launch
+     * without copying them to the host first. */
+    const double *law;
 } agx_multipole;
 
 /* Potential / force / force derivatives at N points.
@@ -180,14 +185,19 @@ int agx_multipole_pack(const agx_multipole *mp, void *dst, int64_t size);
  * NULL) writes dL/dx into d_gxyz [N][3] (may be NULL; a Hessian-vector
  * product) and ADDS dL/d(poly_km) into d_gpoly [n_int][n_lm][6] and
  * dL/d(invPhi0, inner_U, inner_W, outer_U, outer_W) into d_gscal [5]; the
- * caller zeroes those first.  Table gradients are reduced across each
- * wavefront before one FP64 atomic add, so their last bits depend on the
- * order of the atomics.  Needs n_int <= 127.  No reference counterpart (the
+ * caller zeroes those first.  d_work is device scratch of at least
+ * agx_multipole_adjoint_work_size(N) bytes (about 9 N; the particles are
+ * sorted by radial interval there); it is overwritten and needs no clearing.
+ * Table gradients are summed per wavefront and interval before FP64 atomic
+ * adds, so their last bits depend on the order of the atomics.  Needs
+ * n_int <= 127 and N < 2^31 - 8192.  No reference counterpart (the
  * reference has no derivatives with respect to positions or tables); it
  * serves torch.autograd for fitting potentials to tracer forces. */
 int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xyz, int64_t n,
                           const double *d_gphi, const double *d_gforce, double *d_gxyz,
-                          double *d_gpoly, double *d_gscal, void *stream);
+                          double *d_gpoly, double *d_gscal, void *d_work, int64_t work_bytes,
+                          void *stream);
+int64_t agx_multipole_adjoint_work_size(int64_t n);
 
 /* Reverse mode of agx_cylspline_eval's potential+force outputs.  Given
  * dL/dPhi (d_gphi, [N], may be NULL) and dL/dF (d_gforce, [N][3], may be

@@ -176,3 +176,36 @@ def test_knot_gradient(t):
         fd = (vals[0] - vals[1]) / (2 * eps)
         scale = max(abs(fd), np.abs(g).max())
         assert abs(g[i, j] - fd) <= 1e-6 * scale, (i, j, g[i, j], fd)
+
+
+def test_gradient_additive_over_batches():
+    """Large N (runs of several chunks per wavefront, slabs flushed across
+    intervals): the table gradient of the whole set equals the sum over
+    three unequal batches, and dL/dx matches row by row."""
+    from agama_gpu_amd.autograd import FittableMultipole
+    c = _coefs("fire_dense")
+    n = 600_000
+    x = _points(c, n=n, seed=7)
+    print(f"case fire_dense n={n}", flush=True)
+    m = FittableMultipole(c, device=DEV)
+    rng = np.random.default_rng(8)
+    xyz = torch.as_tensor(x, device=DEV)
+    wphi = torch.as_tensor(rng.normal(size=n), device=DEV)
+    wF = torch.as_tensor(rng.normal(size=(n, 3)), device=DEV)
+
+    def grads(sl):
+        xs = xyz[sl].clone().requires_grad_(True)
+        return torch.autograd.grad(_loss(m, xs, wphi[sl], wF[sl]), [m.phi_table, m.dphi_table, xs])
+
+    gP, gD, gx = grads(slice(0, n))
+    parts = [slice(0, 100), slice(100, 250_000), slice(250_000, n)]
+    acc = [torch.zeros_like(gP), torch.zeros_like(gD)]
+    gx_parts = []
+    for sl in parts:
+        a, b, gxs = grads(sl)
+        acc[0] += a
+        acc[1] += b
+        gx_parts.append(gxs)
+    for full, summed in zip((gP, gD), acc):
+        assert (torch.max(torch.abs(full - summed)) / torch.max(torch.abs(full))).item() < 1e-11
+    assert torch.equal(gx, torch.cat(gx_parts))

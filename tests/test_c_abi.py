@@ -55,3 +55,20 @@ def test_argument_validation_without_gpu():
     assert L.agx_multipole_eval(C.byref(d), 8, 10, 8, None, None, 0, None) == -5
     assert L.agx_multipole_eval(C.byref(d), 8, 10, None, None, None, 1, None) == -1
     assert L.agx_multipole_eval(C.byref(d), 8, 0, 8, None, None, 1, None) == 0
+
+
+def test_adjoint_workspace_validation_without_gpu():
+    """The Multipole adjoint needs its documented workspace (about 9 N bytes)."""
+    from agama_gpu_amd import _lib
+    L = _lib.load()
+    n = 1000
+    need = L.agx_multipole_adjoint_work_size(n)
+    assert need >= 9 * n and L.agx_multipole_adjoint_work_size(-1) == 0
+    d = _lib.agx_multipole()
+    d.poly_km = d.program = 8
+    d.n_int, d.n_lm, d.lmax = 4, 1, 0
+    args = (C.byref(d), 8, n, 8, None, None, 8, 8)
+    assert L.agx_multipole_adjoint(*args, None, need, None) == -1
+    assert L.agx_multipole_adjoint(*args, 8, need - 1, None) == -7
+    d.n_int = 128                       # n_int + 1 bins > 128
+    assert L.agx_multipole_adjoint(*args, 8, need, None) == -2
