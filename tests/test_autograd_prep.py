@@ -110,3 +110,17 @@ def test_cylspline_table_gradient_is_local_in_m():
     assert np.isfinite(g).all()
     touched = set(np.nonzero(np.abs(g).max(axis=(1, 2)))[0].tolist())
     assert touched <= {0, k} and k in touched
+
+
+@pytest.mark.parametrize("n", [100, 4096, 10007])
+def test_shift_backward_centre_sum(n):
+    """The shift's centre gradient (blocked column sum) is -sum_i g_i."""
+    from agama_gpu_amd.autograd import _Shift
+    rng = np.random.default_rng(n)
+    x = torch.as_tensor(rng.normal(size=(n, 3)), dtype=torch.float64)
+    c = torch.as_tensor(rng.normal(size=3), dtype=torch.float64).requires_grad_(True)
+    g = torch.as_tensor(rng.normal(size=(n, 3)), dtype=torch.float64)
+    out = _Shift.apply(x, c)
+    assert torch.equal(out, x - c.detach()[None, :])
+    (gc,) = torch.autograd.grad(out, c, g)
+    assert torch.allclose(gc, -g.sum(0), rtol=1e-12, atol=1e-12)
