@@ -107,7 +107,8 @@ _SIGNATURES = {
                                         C.c_void_p, C.c_void_p]),
     "agx_cylspline_adjoint": (C.c_int, [C.POINTER(agx_cylspline), C.c_void_p, C.c_int64,
                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                        C.c_void_p, C.c_void_p]),
+                                        C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    "agx_cylspline_adjoint_work_size": (C.c_int64, [C.POINTER(agx_cylspline), C.c_int64]),
     "agx_cylspline_pack_size": (C.c_int64, []),
     "agx_cylspline_pack": (C.c_int, [C.POINTER(agx_cylspline), C.c_void_p, C.c_int64]),
     "agx_multipole_pack_size": (C.c_int64, []),

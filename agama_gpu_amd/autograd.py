@@ -299,10 +299,14 @@ class _CylSplineEval(torch.autograd.Function):
         gW = torch.zeros_like(W)
         gphi = None if gphi is None else gphi.contiguous()
         gforce = None if gforce is None else gforce.contiguous()
-        rc = _lib.load().agx_cylspline_adjoint(
+        L = _lib.load()
+        nwork = int(L.agx_cylspline_adjoint_work_size(C.byref(desc), n))
+        work = torch.empty(max(nwork, 1), dtype=torch.uint8, device=xyz.device)
+        rc = L.agx_cylspline_adjoint(
             C.byref(desc), xyz.data_ptr(), n, None if gphi is None else gphi.data_ptr(),
             None if gforce is None else gforce.data_ptr(), None if gx is None else gx.data_ptr(),
-            gnodes.data_ptr(), gW.data_ptr(), _lib.stream_handle(xyz.device))
+            gnodes.data_ptr(), gW.data_ptr(), work.data_ptr(), nwork,
+            _lib.stream_handle(xyz.device))
         _lib.check(rc, "agx_cylspline_adjoint")
         return gx, gnodes, gW, None
 

@@ -16,108 +16,24 @@
 // consecutive chunks from a queue counter.  No reference counterpart: the
 // reference has no derivatives with respect to its tables.
 #include "multipole_dev.h"
+#include "sort_dev.h"
 
 namespace agx {
 
 constexpr int kAdjBlock = 256;
-constexpr int64_t kAdjHeader = 2048;     // bytes of counters at the start of the workspace
 // LDS slabs are used while 4 of them (one per wavefront) fit in this many bytes
 constexpr int kAdjSlabLds = 53 * 1024;
 
-// The workspace: counters, each particle's bin, the sorted particle indices.
-struct AdjWork {
-    int *hist;      // [kMaxBins] particles per bin
-    int *off;       // [kMaxBins + 1] first slot of each bin; bins padded to 64 slots
-    int *cursor;    // [kMaxBins] next free slot of each bin while scattering
-    int *queue;     // [1] next run of chunks
-    uint8_t *bin;   // [n]
-    int *idx;       // [n + 64 kMaxBins] particle of each slot
+// Sort key: 0 outside the grid (origin, inner and outer laws), k + 1 in interval k.
+struct MpBinKey {
+    MPK P;
+    const double *xyz;
+    __device__ int operator()(int64_t i) const
+    {
+        const Where w = classify(P, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+        return (w.kind == 2) ? w.k + 1 : 0;
+    }
 };
-
-inline int64_t adj_work_size(int64_t n)
-{
-    return kAdjHeader + ((n + 255) & ~(int64_t)255) + 4 * (n + 64 * (int64_t)kMaxBins);
-}
-
-inline AdjWork adj_work(void *base, int64_t n)
-{
-    AdjWork w;
-    int *h = (int *)base;
-    w.hist = h;
-    w.off = h + kMaxBins;
-    w.cursor = h + 2 * kMaxBins + 1;
-    w.queue = h + 3 * kMaxBins + 1;
-    w.bin = (uint8_t *)base + kAdjHeader;
-    w.idx = (int *)((uint8_t *)base + kAdjHeader + ((n + 255) & ~(int64_t)255));
-    return w;
-}
-
-__device__ __forceinline__ int bin_of(const MPK &P, double x, double y, double z)
-{
-    const Where w = classify(P, x, y, z);
-    return (w.kind == 2) ? w.k + 1 : 0;
-}
-
-__global__ __launch_bounds__(kAdjBlock) void adj_init_kernel(AdjWork W)
-{
-    for (int b = threadIdx.x; b < kMaxBins; b += kAdjBlock) W.hist[b] = 0;
-    if (threadIdx.x == 0) W.queue[0] = 0;
-}
-
-// Bin of every particle, and the histogram (LDS per workgroup, then global).
-__global__ __launch_bounds__(kAdjBlock) void adj_count_kernel(MPK P, const double *__restrict__ xyz,
-                                                             int64_t n, AdjWork W)
-{
-    __shared__ int cnt[kMaxBins];
-    const int nb = P.n_int + 1;
-    for (int b = threadIdx.x; b < nb; b += kAdjBlock) cnt[b] = 0;
-    __syncthreads();
-    for (int64_t i = (int64_t)blockIdx.x * kAdjBlock + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kAdjBlock) {
-        const int b = bin_of(P, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
-        W.bin[i] = (uint8_t)b;
-        atomicAdd(&cnt[b], 1);
-    }
-    __syncthreads();
-    for (int b = threadIdx.x; b < nb; b += kAdjBlock)
-        if (cnt[b]) atomicAdd(&W.hist[b], cnt[b]);
-}
-
-// Exclusive scan of the 64-padded bin sizes (one thread: at most 128 bins).
-__global__ void adj_scan_kernel(AdjWork W, int nb)
-{
-    if (threadIdx.x != 0) return;
-    int o = 0;
-    for (int b = 0; b < nb; ++b) {
-        W.off[b] = o;
-        W.cursor[b] = o;
-        o += (W.hist[b] + 63) & ~63;
-    }
-    W.off[nb] = o;
-}
-
-// Particle indices into their bins' slots: each workgroup reserves one range
-// per bin (one global atomic), then places its particles with LDS counters.
-// Same particle-to-workgroup mapping as adj_count_kernel.
-__global__ __launch_bounds__(kAdjBlock) void adj_scatter_kernel(int64_t n, int nb, AdjWork W)
-{
-    __shared__ int cnt[kMaxBins], base[kMaxBins];
-    for (int b = threadIdx.x; b < nb; b += kAdjBlock) cnt[b] = 0;
-    __syncthreads();
-    const int64_t i0 = (int64_t)blockIdx.x * kAdjBlock + threadIdx.x;
-    const int64_t stride = (int64_t)gridDim.x * kAdjBlock;
-    for (int64_t i = i0; i < n; i += stride) atomicAdd(&cnt[W.bin[i]], 1);
-    __syncthreads();
-    for (int b = threadIdx.x; b < nb; b += kAdjBlock) {
-        base[b] = cnt[b] ? atomicAdd(&W.cursor[b], cnt[b]) : 0;
-        cnt[b] = 0;
-    }
-    __syncthreads();
-    for (int64_t i = i0; i < n; i += stride) {
-        const int b = W.bin[i];
-        W.idx[base[b] + atomicAdd(&cnt[b], 1)] = (int)i;
-    }
-}
 
 // s^j a + j s^(j-1) b for j = 0..5, summed over the wavefront and added to
 // g[0..5]: a wavefront-private LDS slab (SLAB) or the table gradient in memory.
@@ -301,13 +217,13 @@ __device__ __forceinline__ void flush_slab(double *slab, double *g, int cnt, int
     __builtin_amdgcn_wave_barrier();
 }
 
-// Runs of run_len consecutive 64-slot chunks of the sorted order, taken from
-// the queue by whole wavefronts until none is left.
+// The sorted order, 64 particles of one bin per wavefront and chunk.  Three
+// wavefronts per SIMD (at most 168 VGPRs).
 template <bool SLAB>
-__global__ __launch_bounds__(kAdjBlock) void mp_adjoint_kernel(
+__global__ __launch_bounds__(kAdjBlock, 3) void mp_adjoint_kernel(
     MPK P, const double *__restrict__ xyz, const double *__restrict__ gphi,
     const double *__restrict__ gforce, double *__restrict__ gxyz, double *__restrict__ gpoly,
-    double *__restrict__ gscal, AdjWork W, int run_len)
+    double *__restrict__ gscal, SortWork W, int run_len)
 {
     P = with_law(P);
     extern __shared__ double adj_lds[];
@@ -316,48 +232,27 @@ __global__ __launch_bounds__(kAdjBlock) void mp_adjoint_kernel(
     double *slab = adj_lds + (SLAB ? (threadIdx.x >> 6) * slab_n : 0);
     if (SLAB)
         for (int q = lane; q < slab_n; q += 64) slab[q] = 0.0;
-    const int nb = P.n_int + 1;
-    const int total = cldi(W.off, nb) >> 6;
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};   // invPhi0, inner U, W, outer U, W
     int cur = -1;                                // interval + 1 held by the slab
-    for (;;) {
-        int run = 0;
-        if (lane == 0) run = atomicAdd(W.queue, 1);
-        run = __builtin_amdgcn_readfirstlane(run);
-        const int c0 = run * run_len;
-        if (c0 >= total) break;
-        const int c1 = min(c0 + run_len, total);
-        // bin of chunk c0: the last b with off[b] <= 64 c0 (off[0] = 0 <= 64 c0 < off[nb])
-        int b = 0, hi = nb;
-        while (hi - b > 1) {
-            const int mid = (b + hi) >> 1;
-            if (cldi(W.off, mid) <= c0 * 64) b = mid; else hi = mid;
+    for_each_sorted_chunk(W, P.n_int + 1, run_len, [&](int b, bool valid, int64_t i) {
+        if (SLAB && b != cur) {
+            if (cur > 0) flush_slab(slab, gpoly + (int64_t)(cur - 1) * slab_n, slab_n, lane);
+            cur = b;
         }
-        for (int c = c0; c < c1; ++c) {
-            while (cldi(W.off, b + 1) <= c * 64) ++b;
-            if (SLAB && b != cur) {
-                if (cur > 0) flush_slab(slab, gpoly + (int64_t)(cur - 1) * slab_n, slab_n, lane);
-                cur = b;
-            }
-            const int sl = c * 64 + lane;
-            const bool valid = sl < cldi(W.off, b) + cldi(W.hist, b);
-            double x = 1.0, y = 0.0, z = 0.0, wphi = 0.0, wf[3] = {0.0, 0.0, 0.0};
-            int64_t i = 0;
-            if (valid) {
-                i = W.idx[sl];
-                x = xyz[3 * i]; y = xyz[3 * i + 1]; z = xyz[3 * i + 2];
-                if (gphi) wphi = gphi[i];
-                if (gforce) { wf[0] = gforce[3 * i]; wf[1] = gforce[3 * i + 1]; wf[2] = gforce[3 * i + 2]; }
-            }
-            double gx[3] = {0.0, 0.0, 0.0};
-            if (b > 0)
-                adjoint_interior<SLAB>(P, b - 1, valid, x, y, z, wphi, wf, gx,
-                                       SLAB ? slab : gpoly + (int64_t)(b - 1) * slab_n, acc[0], lane);
-            else
-                adjoint_extrap(P, valid, x, y, z, wphi, wf, gx, acc + 1);
-            if (valid && gxyz) { gxyz[3 * i] = gx[0]; gxyz[3 * i + 1] = gx[1]; gxyz[3 * i + 2] = gx[2]; }
+        double x = 1.0, y = 0.0, z = 0.0, wphi = 0.0, wf[3] = {0.0, 0.0, 0.0};
+        if (valid) {
+            x = xyz[3 * i]; y = xyz[3 * i + 1]; z = xyz[3 * i + 2];
+            if (gphi) wphi = gphi[i];
+            if (gforce) { wf[0] = gforce[3 * i]; wf[1] = gforce[3 * i + 1]; wf[2] = gforce[3 * i + 2]; }
         }
-    }
+        double gx[3] = {0.0, 0.0, 0.0};
+        if (b > 0)
+            adjoint_interior<SLAB>(P, b - 1, valid, x, y, z, wphi, wf, gx,
+                                   SLAB ? slab : gpoly + (int64_t)(b - 1) * slab_n, acc[0], lane);
+        else
+            adjoint_extrap(P, valid, x, y, z, wphi, wf, gx, acc + 1);
+        if (valid && gxyz) { gxyz[3 * i] = gx[0]; gxyz[3 * i + 1] = gx[1]; gxyz[3 * i + 2] = gx[2]; }
+    });
     if (SLAB && cur > 0) flush_slab(slab, gpoly + (int64_t)(cur - 1) * slab_n, slab_n, lane);
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
@@ -372,7 +267,7 @@ using namespace agx;
 
 extern "C" int64_t agx_multipole_adjoint_work_size(int64_t n)
 {
-    return n < 0 ? 0 : adj_work_size(n);
+    return n < 0 ? 0 : sort_work_size(n, kMaxBins);
 }
 
 extern "C" int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xyz, int64_t n,
@@ -387,22 +282,15 @@ extern "C" int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xy
     if (mp->lmax > 32) return AGX_E_LMAX;
     if (n == 0) return AGX_OK;
     if (!d_xyz || (!d_gphi && !d_gforce) || !d_work) return AGX_E_NULL;
-    if (work_bytes < adj_work_size(n)) return AGX_E_WORKSPACE;
+    if (work_bytes < sort_work_size(n, kMaxBins)) return AGX_E_WORKSPACE;
     MPK P = make_args(mp);
-    const AdjWork W = adj_work(d_work, n);
+    const SortWork W = sort_work(d_work, n, kMaxBins);
     const int nb = mp->n_int + 1;
     hipStream_t st = (hipStream_t)stream;
-    const int64_t n_blk = (n + kAdjBlock - 1) / kAdjBlock;
-    const unsigned sort_grid = (unsigned)(n_blk < 1024 ? n_blk : 1024);
-    hipLaunchKernelGGL(adj_init_kernel, dim3(1), dim3(kAdjBlock), 0, st, W);
-    hipLaunchKernelGGL(adj_count_kernel, dim3(sort_grid), dim3(kAdjBlock), 0, st, P, d_xyz, n, W);
-    hipLaunchKernelGGL(adj_scan_kernel, dim3(1), dim3(64), 0, st, W, nb);
-    hipLaunchKernelGGL(adj_scatter_kernel, dim3(sort_grid), dim3(kAdjBlock), 0, st, n, nb, W);
-    // at most this many chunks; runs of run_len chunks, about two per resident wavefront
+    sort_launch(MpBinKey{P, d_xyz}, n, nb, W, st);
+    // about two runs per resident wavefront (3 per SIMD)
+    const int run_len = sort_run_len(n, nb, 256LL * 12);
     const int64_t chunks = (n + 63) / 64 + nb;
-    const int64_t waves = 256LL * 12;
-    int run_len = (int)((chunks + 2 * waves - 1) / (2 * waves));
-    if (run_len < 1) run_len = 1;
     const int64_t wg_need = (chunks + run_len - 1) / run_len / (kAdjBlock / 64) + 1;
     const unsigned grid = (unsigned)(wg_need < 1024 ? wg_need : 1024);
     const size_t slab_bytes = (size_t)mp->n_lm * 6 * sizeof(double) * (kAdjBlock / 64);

@@ -228,6 +228,22 @@ __device__ __forceinline__ void wave_sum6(double v0, double v1, double v2, doubl
     e = row_sum(c + c2);
 }
 
+// Transposed wavefront sums of v[0..15]: on return, row r of d[k] holds the
+// total of v[4k + (((r & 1) << 1) | (r >> 1))].  Every lane must be active.
+__device__ __forceinline__ void wave_sum16(double v[16], double d[4])
+{
+#pragma unroll
+    for (int j = 0; j < 8; ++j) lane_swap<false>(v[2 * j], v[2 * j + 1]);
+    double a[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a[j] = v[2 * j] + v[2 * j + 1];   // lo: v[2j], hi: v[2j+1]
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        lane_swap<true>(a[2 * k], a[2 * k + 1]);
+        d[k] = row_sum(a[2 * k] + a[2 * k + 1]);
+    }
+}
+
 // wave_sum6, added to g[0..5] by FP64 atomics (two atomic instructions, 4 and 2 lanes).
 __device__ __forceinline__ void wave_sum6_add(double *g, double v0, double v1, double v2,
                                               double v3, double v4, double v5, int lane)

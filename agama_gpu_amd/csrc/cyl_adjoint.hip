@@ -1,20 +1,24 @@
 // Adjoint (reverse-mode) kernel of the CylSpline potential+force evaluation.
 //
 // Given upstream gradients w_phi = dL/dPhi_i and w_F = dL/dF_i (F = -grad Phi)
-// at N points, one launch produces
+// at N points, one call produces
 //   dL/dx_i      = w_phi grad Phi - H w_F            (Hessian-vector product)
 //   dL/dnodes    [n_harm][nR][nz][4]  (f, f_lR, f_lz, f_lRlz of every node)
 //   dL/dW        [(lmax_outer+1)^2]   (the outer PowerLaw coefficients)
 // Inside the grid the value of harmonic m at a point is linear in the 16 node
 // quantities of its cell, with the bicubic Hermite weights A_a(tR) B_b(tz)
 // (value nodes), A_a B'_b (lz slopes), A'_a B_b (lR slopes), A'_a B'_b
-// (cross slopes); the gradient with respect to the harmonic's value and its
-// lR, lz derivatives is spread onto them with one FP64 atomic per node
-// quantity (cells differ from lane to lane).  Outside the grid Phi is linear
-// in W; each W entry's contribution is summed over the wavefront before one
-// atomic.  No reference counterpart (the reference has no derivatives with
-// respect to positions or tables).
+// (cross slopes).  The particles are first sorted by grid cell (a counting
+// sort into the caller's workspace), so the 64 lanes of a wavefront share a
+// cell: the 16 gradients of each harmonic are summed across the wavefront
+// (lane swaps) and added to memory by 16 lanes, one FP64 atomic each.
+// Outside the grid Phi is linear in W; each W entry's contribution is summed
+// over the wavefront before one atomic.  Grids of more than 8191 cells fall
+// back to one atomic per particle and node quantity.  No reference
+// counterpart (the reference has no derivatives with respect to positions or
+// tables).
 #include "cylspline_dev.h"
+#include "sort_dev.h"
 
 namespace agx {
 
@@ -96,11 +100,17 @@ __device__ __forceinline__ void cyl_adjoint_outer(const CSK &P, bool valid, doub
     }
 }
 
+// One point inside the grid.  SORTED: every lane of the wavefront is in cell
+// (iR, iz) (wavefront-uniform) and the node gradients are summed across the
+// wavefront before the atomics; otherwise the cell is found per lane and each
+// lane adds its own 16 gradients per harmonic.
+template <bool SORTED>
 __device__ __forceinline__ void cyl_adjoint_inner(const CSK &P, double x, double y, double z,
                                                   double R, double lR, double lz, double wphi,
-                                                  const double wf[3], double *gnodes)
+                                                  const double wf[3], double *gnodes, int iR,
+                                                  int iz)
 {
-    const int iR = cell_search(lR, P.lR, P.nR), iz = cell_search(lz, P.lz, P.nz);
+    if (!SORTED) { iR = cell_search(lR, P.lR, P.nR); iz = cell_search(lz, P.lz, P.nz); }
     const double lR0 = cld(P.lR, iR), lR1 = cld(P.lR, iR + 1);
     const double lz0 = cld(P.lz, iz), lz1 = cld(P.lz, iz + 1);
     const double hR = lR1 - lR0, hz = lz1 - lz0;
@@ -139,6 +149,29 @@ __device__ __forceinline__ void cyl_adjoint_inner(const CSK &P, double x, double
     };
     auto scatter = [&](int mm, double gv, double gvR, double gvZ) {
         double *gd = gnodes + ((int64_t)mm * plane + (int64_t)iR * P.nz + iz) * 4;
+        if constexpr (SORTED) {
+            double v[16], d[4];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    const double Aa = A[a], Ada = A[2 + a], Bb = B[b], Bdb = B[2 + b];
+                    const double dAa = dA[a], dAda = dA[2 + a], dBb = dB[b], dBdb = dB[2 + b];
+                    double *q = v + (2 * a + b) * 4;
+                    q[0] = gv * Aa * Bb + gvR * dAa * Bb + gvZ * Aa * dBb;
+                    q[1] = gv * Ada * Bb + gvR * dAda * Bb + gvZ * Ada * dBb;
+                    q[2] = gv * Aa * Bdb + gvR * dAa * Bdb + gvZ * Aa * dBdb;
+                    q[3] = gv * Ada * Bdb + gvR * dAda * Bdb + gvZ * Ada * dBdb;
+                }
+            wave_sum16(v, d);
+            const int lane = threadIdx.x & 63;
+            if ((lane & 15) == 0) {
+                const int r = lane >> 4, q = ((r & 1) << 1) | (r >> 1);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) atomicAdd(gd + ((c >> 1) * P.nz + (c & 1)) * 4 + q, d[c]);
+            }
+            return;
+        }
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -230,22 +263,91 @@ __global__ __launch_bounds__(kBlock) void cyl_adjoint_kernel(
         const double lR = asinh(R * iRs), lz = asinh(z * iRs);
         const bool outside = lR < cld(P.lR, 0) || lR > cld(P.lR, P.nR - 1) ||
                              lz < cld(P.lz, 0) || lz > cld(P.lz, P.nz - 1);
-        if (valid && !outside) cyl_adjoint_inner(P, x, y, z, R, lR, lz, wphi, wf, gnodes);
+        if (valid && !outside) cyl_adjoint_inner<false>(P, x, y, z, R, lR, lz, wphi, wf, gnodes, 0, 0);
         // the outer sums run for every wavefront that has an outside point
         if (__ballot(valid && outside)) cyl_adjoint_outer(P, valid && outside, x, y, z, R, wphi, wf, gW);
     }
+}
+
+// Sort key: 0 outside the grid, 1 + iR (nz - 1) + iz in cell (iR, iz).
+struct CylCellKey {
+    CSK P;
+    const double *xyz;
+    __device__ int operator()(int64_t i) const
+    {
+        const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+        const double iRs = 1.0 / P.Rscale;
+        const double lR = asinh(sqrt(fma(x, x, y * y)) * iRs), lz = asinh(z * iRs);
+        if (lR < cld(P.lR, 0) || lR > cld(P.lR, P.nR - 1) || lz < cld(P.lz, 0) ||
+            lz > cld(P.lz, P.nz - 1))
+            return 0;
+        return 1 + cell_search(lR, P.lR, P.nR) * (P.nz - 1) + cell_search(lz, P.lz, P.nz);
+    }
+};
+
+// The sorted order: 64 particles of one cell (or of the outside) per chunk.
+__global__ __launch_bounds__(kBlock) void cyl_adjoint_sorted_kernel(
+    CSK P, const double *__restrict__ xyz, const double *__restrict__ gphi,
+    const double *__restrict__ gforce, double *__restrict__ gxyz, double *__restrict__ gnodes,
+    double *__restrict__ gW, SortWork W, int nb, int run_len)
+{
+    for_each_sorted_chunk(W, nb, run_len, [&](int b, bool valid, int64_t i) {
+        double x = 1.0e30, y = 0.0, z = 0.0, wphi = 0.0, wf[3] = {0.0, 0.0, 0.0};
+        int iR = 0, iz = 0;
+        if (b > 0) {
+            iR = (b - 1) / (P.nz - 1);
+            iz = (b - 1) - iR * (P.nz - 1);
+            // padding lanes: the cell's centre, with zero seeds
+            const double lRc = 0.5 * (cld(P.lR, iR) + cld(P.lR, iR + 1));
+            const double lzc = 0.5 * (cld(P.lz, iz) + cld(P.lz, iz + 1));
+            x = P.Rscale * sinh(lRc); z = P.Rscale * sinh(lzc);
+        }
+        if (valid) {
+            x = xyz[3 * i]; y = xyz[3 * i + 1]; z = xyz[3 * i + 2];
+            if (gphi) wphi = gphi[i];
+            if (gforce) { wf[0] = gforce[3 * i]; wf[1] = gforce[3 * i + 1]; wf[2] = gforce[3 * i + 2]; }
+            if (gxyz) {
+                PointOut o{};
+                cylspline_point<ORD_HESS>(P, x, y, z, o);
+                gxyz[3 * i] = wphi * o.g[0] - (o.h[0] * wf[0] + o.h[3] * wf[1] + o.h[5] * wf[2]);
+                gxyz[3 * i + 1] = wphi * o.g[1] - (o.h[3] * wf[0] + o.h[1] * wf[1] + o.h[4] * wf[2]);
+                gxyz[3 * i + 2] = wphi * o.g[2] - (o.h[5] * wf[0] + o.h[4] * wf[1] + o.h[2] * wf[2]);
+            }
+        }
+        const double R = sqrt(fma(x, x, y * y));
+        if (b > 0) {
+            const double iRs = 1.0 / P.Rscale;
+            const double lR = asinh(R * iRs), lz = asinh(z * iRs);
+            cyl_adjoint_inner<true>(P, x, y, z, R, lR, lz, wphi, wf, gnodes, iR, iz);
+        } else {
+            cyl_adjoint_outer(P, valid, x, y, z, R, wphi, wf, gW);
+        }
+    });
+}
+
+inline int cyl_bins(const agx_cylspline *cs)
+{
+    return (cs->nR - 1) * (cs->nz - 1) + 1;
 }
 
 }  // namespace agx
 
 using namespace agx;
 
+extern "C" int64_t agx_cylspline_adjoint_work_size(const agx_cylspline *cs, int64_t n)
+{
+    if (!cs || n < 0 || cs->nR < 2 || cs->nz < 2) return 0;
+    const int nb = cyl_bins(cs);
+    return nb <= kSortMaxBins ? sort_work_size(n, nb) : 0;
+}
+
 extern "C" int agx_cylspline_adjoint(const agx_cylspline *cs, const double *d_xyz, int64_t n,
                                      const double *d_gphi, const double *d_gforce, double *d_gxyz,
-                                     double *d_gnodes, double *d_gW, void *stream)
+                                     double *d_gnodes, double *d_gW, void *d_work,
+                                     int64_t work_bytes, void *stream)
 {
     if (!cs) return AGX_E_NULL;
-    if (n < 0) return AGX_E_SIZE;
+    if (n < 0 || n > (int64_t)0x7FFFFFFF - 64 * (int64_t)kSortMaxBins) return AGX_E_SIZE;
     if (!cs->lR_grid || !cs->lz_grid || !cs->nodes || !cs->m_arr || !cs->W_outer || !d_gnodes ||
         !d_gW)
         return AGX_E_NULL;
@@ -253,8 +355,23 @@ extern "C" int agx_cylspline_adjoint(const agx_cylspline *cs, const double *d_xy
     if (n == 0) return AGX_OK;
     if (!d_xyz || (!d_gphi && !d_gforce)) return AGX_E_NULL;
     CSK P = make_cyl_args(cs);
-    const int64_t n_pad = (n + 63) & ~(int64_t)63;
-    hipLaunchKernelGGL(cyl_adjoint_kernel, dim3(grid_for(n_pad)), dim3(kBlock), 0,
-                       (hipStream_t)stream, P, d_xyz, n, d_gphi, d_gforce, d_gxyz, d_gnodes, d_gW);
+    hipStream_t st = (hipStream_t)stream;
+    const int nb = cyl_bins(cs);
+    if (nb <= kSortMaxBins) {
+        if (!d_work) return AGX_E_NULL;
+        if (work_bytes < sort_work_size(n, nb)) return AGX_E_WORKSPACE;
+        const SortWork W = sort_work(d_work, n, nb);
+        sort_launch(CylCellKey{P, d_xyz}, n, nb, W, st);
+        const int run_len = sort_run_len(n, nb, 256LL * 8);
+        const int64_t chunks = (n + 63) / 64 + nb;
+        const int64_t wg_need = (chunks + run_len - 1) / run_len / (kBlock / 64) + 1;
+        const unsigned grid = (unsigned)(wg_need < 1024 ? wg_need : 1024);
+        hipLaunchKernelGGL(cyl_adjoint_sorted_kernel, dim3(grid), dim3(kBlock), 0, st, P, d_xyz,
+                           d_gphi, d_gforce, d_gxyz, d_gnodes, d_gW, W, nb, run_len);
+    } else {
+        const int64_t n_pad = (n + 63) & ~(int64_t)63;
+        hipLaunchKernelGGL(cyl_adjoint_kernel, dim3(grid_for(n_pad)), dim3(kBlock), 0, st, P, d_xyz,
+                           n, d_gphi, d_gforce, d_gxyz, d_gnodes, d_gW);
+    }
     return hipGetLastError() == hipSuccess ? AGX_OK : AGX_E_LAUNCH;
 }

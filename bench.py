@@ -285,6 +285,8 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-ensemble and gradient legs")
+    ap.add_argument("--only", choices=["dense", "analytic", "batched", "grad", "leapfrog"],
+                    help="run one secondary leg alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy three times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known traffic)")
@@ -308,6 +310,12 @@ def main():
     from agama_gpu_amd import MultipolePotentialGPU
     from agama_gpu_amd.workloads import halo_lmax8_coefs, hernquist_points
 
+    if args.only in ("batched", "grad", "leapfrog"):
+        dev = torch.device("cuda", 0)
+        leg = {"batched": bench_batched, "grad": bench_grad, "leapfrog": bench_leapfrog}[args.only]
+        print(json.dumps({args.only: leg(dev)}), flush=True)
+        return
+
     N = args.n
     coefs = halo_lmax8_coefs()
     x_np = hernquist_points(N, 0)
@@ -321,6 +329,10 @@ def main():
     phi = torch.empty(N, dtype=torch.float64, device=dev)
     force = torch.empty((N, 3), dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
+    if args.only:
+        leg = {"dense": bench_dense, "analytic": bench_analytic}[args.only]
+        print(json.dumps({args.only: leg(dev, x_np, xyz, phi, force)}), flush=True)
+        return
 
     def step():
         pot.eval_into(xyz, phi=phi, force=force)

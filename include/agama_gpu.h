@@ -186,7 +186,7 @@ int agx_multipole_pack(const agx_multipole *mp, void *dst, int64_t size);
  * product) and ADDS dL/d(poly_km) into d_gpoly [n_int][n_lm][6] and
  * dL/d(invPhi0, inner_U, inner_W, outer_U, outer_W) into d_gscal [5]; the
  * caller zeroes those first.  d_work is device scratch of at least
- * agx_multipole_adjoint_work_size(N) bytes (about 9 N; the particles are
+ * agx_multipole_adjoint_work_size(N) bytes (about 10 N; the particles are
  * sorted by radial interval there); it is overwritten and needs no clearing.
  * Table gradients are summed per wavefront and interval before FP64 atomic
  * adds, so their last bits depend on the order of the atomics.  Needs
@@ -204,14 +204,20 @@ int64_t agx_multipole_adjoint_work_size(int64_t n);
  * NULL) writes dL/dx into d_gxyz [N][3] (may be NULL; a Hessian-vector
  * product) and ADDS dL/d(nodes) into d_gnodes [n_harm][nR][nz][4] and
  * dL/d(W_outer) into d_gW [(lmax_outer+1)^2]; the caller zeroes those first.
- * Node gradients take one FP64 atomic add per touched node quantity, outer
- * coefficients one per wavefront, so their last bits depend on the order of
- * the atomics.  No reference counterpart (the reference has no derivatives
- * with respect to positions or tables); it serves torch.autograd for fitting
- * disc potentials to tracer forces. */
+ * d_work is device scratch of agx_cylspline_adjoint_work_size(cs, N) bytes
+ * (about 6 N + 256 per grid cell: the particles are sorted by grid cell
+ * there); overwritten, needs no clearing.  A size of 0 means the grid has
+ * more than 8191 cells: then d_work may be NULL and every particle adds its
+ * own node gradients.  Gradients are summed per wavefront before FP64 atomic
+ * adds, so their last bits depend on the order of the atomics.  N < 2^31 -
+ * 2^19.  No reference counterpart (the reference has no derivatives with
+ * respect to positions or tables); it serves torch.autograd for fitting disc
+ * potentials to tracer forces. */
 int agx_cylspline_adjoint(const agx_cylspline *cs, const double *d_xyz, int64_t n,
                           const double *d_gphi, const double *d_gforce, double *d_gxyz,
-                          double *d_gnodes, double *d_gW, void *stream);
+                          double *d_gnodes, double *d_gW, void *d_work, int64_t work_bytes,
+                          void *stream);
+int64_t agx_cylspline_adjoint_work_size(const agx_cylspline *cs, int64_t n);
 
 /* ---------------------------------------------------------------------------
  * Ragged ensembles: many independent Multipole potentials, each with its own
