@@ -158,10 +158,17 @@ def bench_batched(dev, reps: int = 5):
             "setup_s": t_tune - t_setup, "tune_s": t_ready - t_tune}
 
 
-def _event_times(fn, reps: int = 10):
+def _event_times(fn, reps: int = 20, settle: float = 0.5):
+    """HIP-event times of `reps` launches of fn, after `settle` seconds of
+    untimed launches: a leg that follows the CPU-baseline leg (seconds with the
+    GPU idle) otherwise starts at low clocks and reads up to 20 % slow."""
     import torch
+    t0 = time.perf_counter()
     fn()
     torch.cuda.synchronize()
+    while time.perf_counter() - t0 < settle:
+        fn()
+        torch.cuda.synchronize()
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(reps)]
     for a, b in evs:
