@@ -13,10 +13,12 @@ algorithmic flops per launch / mean kernel time; HBM traffic per launch from
 the committed rocprof summary) and ``cpu_baseline`` (the oracle's OpenMP C
 restatement on this host's cores, same workload), and the other legs of the
 metric (skipped with --no-extras): ``dense`` (the same points through the
-dense 81-term lmax-8 FIRE table), ``analytic`` (NFW, HBM-bound), ``batched``
-(256-potential ragged ensemble, one launch vs one per potential), ``grad``
-(force-fit forward+backward) and ``leapfrog`` (HIP-graph replay vs per-step
-API calls).
+dense 81-term lmax-8 FIRE table), ``analytic`` (NFW, HBM-bound),
+``composite`` (Milky-Way-like Multipole + CylSpline + NFW + Plummer, evalDeriv
+at N=1M in one fused launch), ``high_order`` (lmax-32 Multipole evalDeriv at
+N=10M), ``batched`` (256-potential ragged ensemble, one launch vs one per
+potential), ``grad`` (force-fit forward+backward) and ``leapfrog`` (HIP-graph
+replay vs per-step API calls).  ``--only LEG`` runs one of them alone.
 """
 from __future__ import annotations
 
@@ -209,6 +211,45 @@ def bench_analytic(dev, xyz, phi, force):
             "frac_hbm_peak": gbs / PEAK_HBM_GBS}
 
 
+def bench_composite(dev, n: int = 1_000_000):
+    """Milky-Way-like composite (Multipole halo lmax 8 + CylSpline disk and
+    bar, 32 x 32 grid, mmax 6 + NFW + Plummer), evalDeriv (phi, force,
+    Hessian) at N=1M Hernquist points in one fused launch."""
+    import torch
+    from agama_gpu_amd.workloads import hernquist_points, mw_composite
+    comp = mw_composite(device=dev)
+    xyz = torch.as_tensor(hernquist_points(n, 11), device=dev)
+    phi = torch.empty(n, dtype=torch.float64, device=dev)
+    force = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    deriv = torch.empty((n, 6), dtype=torch.float64, device=dev)
+    ts = _event_times(lambda: comp.eval_into(xyz, phi=phi, force=force, deriv=deriv))
+    ms = float(np.median(ts))
+    return {"components": "Multipole lmax 8 + CylSpline 32x32 mmax 6 + NFW + Plummer",
+            "n_particles": n, "kernel_ms_median": ms, "particles_per_s": n / (ms * 1e-3),
+            "launches": 1}
+
+
+def bench_high_order(dev, n: int = 10_000_000):
+    """Multipole lmax 32 (80 nodes, 1085 active harmonics), evalDeriv at
+    N=10M points log-uniform in r over [0.05, 800] kpc."""
+    import torch
+    from agama_gpu_amd import MultipolePotentialGPU
+    from agama_gpu_amd.workloads import high_order_coefs, loguniform_points
+    pot = MultipolePotentialGPU(high_order_coefs(), device=dev)
+    x_np = loguniform_points(n, 2, 0.05, 800.0)
+    flops, info = opcount_multipole(pot._data, classify(pot._data, x_np), order=2)
+    xyz = torch.as_tensor(x_np, device=dev)
+    del x_np
+    phi = torch.empty(n, dtype=torch.float64, device=dev)
+    force = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    deriv = torch.empty((n, 6), dtype=torch.float64, device=dev)
+    ts = _event_times(lambda: pot.eval_into(xyz, phi=phi, force=force, deriv=deriv), reps=10)
+    ms = float(np.median(ts))
+    ach = flops / (ms * 1e-3) / 1e12
+    return {"n_particles": n, "kernel_ms_median": ms, "particles_per_s": n / (ms * 1e-3),
+            "achieved_tflops": ach, "frac_fp64_peak": ach / PEAK_FP64_TFLOPS, "opcount": info}
+
+
 def bench_grad(dev, reps: int = 5):
     """Force fit: Multipole lmax 8 (2 x 81 x 50 live table entries) shifted
     along a 200-knot spline centre, N=1M tracers; one step = loss (mean
@@ -292,7 +333,8 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-ensemble and gradient legs")
-    ap.add_argument("--only", choices=["dense", "analytic", "batched", "grad", "leapfrog"],
+    ap.add_argument("--only", choices=["dense", "analytic", "composite", "high_order", "batched",
+                                       "grad", "leapfrog"],
                     help="run one secondary leg alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy three times (calibrates the "
@@ -317,9 +359,10 @@ def main():
     from agama_gpu_amd import MultipolePotentialGPU
     from agama_gpu_amd.workloads import halo_lmax8_coefs, hernquist_points
 
-    if args.only in ("batched", "grad", "leapfrog"):
+    if args.only in ("composite", "high_order", "batched", "grad", "leapfrog"):
         dev = torch.device("cuda", 0)
-        leg = {"batched": bench_batched, "grad": bench_grad, "leapfrog": bench_leapfrog}[args.only]
+        leg = {"composite": bench_composite, "high_order": bench_high_order,
+               "batched": bench_batched, "grad": bench_grad, "leapfrog": bench_leapfrog}[args.only]
         print(json.dumps({args.only: leg(dev)}), flush=True)
         return
 
@@ -395,6 +438,8 @@ def main():
     if not args.no_extras:
         extras["dense"] = bench_dense(dev, x_np, xyz, phi, force)
         extras["analytic"] = bench_analytic(dev, xyz, phi, force)
+        extras["composite"] = bench_composite(dev)
+        extras["high_order"] = bench_high_order(dev)
         extras["batched"] = bench_batched(dev)
         extras["grad"] = bench_grad(dev)
         extras["leapfrog"] = bench_leapfrog(dev)
