@@ -38,6 +38,16 @@ class agx_multipole(C.Structure):
     ]
 
 
+class agx_multipole_source(C.Structure):
+    _fields_ = [
+        ("phi", C.c_void_p), ("dphi", C.c_void_p), ("r", C.c_void_p), ("cols", C.c_void_p),
+        ("Mf", C.c_void_p), ("Md", C.c_void_p),
+        ("nR", C.c_int32), ("n_cols", C.c_int32), ("n_lm", C.c_int32),
+        ("log_scaling", C.c_int32), ("inner_mode", C.c_int32), ("inner_reset", C.c_int32),
+        ("outer_mode", C.c_int32), ("pad", C.c_int32), ("dlogr", C.c_double),
+    ]
+
+
 class agx_cylspline(C.Structure):
     _fields_ = [
         ("lR_grid", C.c_void_p), ("lz_grid", C.c_void_p), ("nodes", C.c_void_p),
@@ -101,6 +111,11 @@ _SIGNATURES = {
                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     "agx_multipole_adjoint_work_size": (C.c_int64, [C.c_int64]),
+    "agx_multipole_build": (C.c_int, [C.POINTER(agx_multipole_source), C.c_void_p, C.c_void_p,
+                                      C.c_void_p]),
+    "agx_multipole_build_adjoint": (C.c_int, [C.POINTER(agx_multipole_source), C.c_void_p,
+                                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_void_p]),
     "agx_cylspline_eval": (C.c_int, [C.POINTER(agx_cylspline), C.c_void_p, C.c_int64,
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "agx_cylspline_density": (C.c_int, [C.POINTER(agx_cylspline), C.c_void_p, C.c_int64,

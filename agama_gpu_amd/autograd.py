@@ -125,6 +125,46 @@ class _MultipoleEval(torch.autograd.Function):
         return gx, gpoly, gscal, None
 
 
+class _MultipoleBuild(torch.autograd.Function):
+    """(phi_table, dphi_table) -> (poly_km, law) by agx_multipole_build on the
+    device, with agx_multipole_build_adjoint as backward: one launch each way
+    instead of the ~100 small torch operations of the same rebuild."""
+
+    @staticmethod
+    def forward(ctx, phi_table, dphi_table, owner):
+        d = owner._d
+        src = owner._source(phi_table, dphi_table)
+        dev = phi_table.device
+        poly = torch.empty((d["n_intervals"], d["n_lm"], 6), dtype=F64, device=dev)
+        law = torch.empty(5, dtype=F64, device=dev)
+        rc = _lib.load().agx_multipole_build(C.byref(src), poly.data_ptr(), law.data_ptr(),
+                                             _lib.stream_handle(dev))
+        _lib.check(rc, "agx_multipole_build")
+        ctx.save_for_backward(phi_table, dphi_table)
+        ctx.owner = owner
+        return poly, law
+
+    @staticmethod
+    def backward(ctx, gpoly, glaw):
+        phi_table, dphi_table = ctx.saved_tensors
+        dev = phi_table.device
+        src = ctx.owner._source(phi_table, dphi_table)
+        d = ctx.owner._d
+        if gpoly is None:
+            gpoly = torch.zeros((d["n_intervals"], d["n_lm"], 6), dtype=F64, device=dev)
+        if glaw is None:
+            glaw = torch.zeros(5, dtype=F64, device=dev)
+        gpoly, glaw = gpoly.contiguous(), glaw.contiguous()
+        gphi = torch.zeros_like(phi_table)
+        gdphi = torch.zeros_like(dphi_table)
+        work = torch.empty(1, dtype=F64, device=dev)
+        rc = _lib.load().agx_multipole_build_adjoint(
+            C.byref(src), gpoly.data_ptr(), glaw.data_ptr(), gphi.data_ptr(), gdphi.data_ptr(),
+            work.data_ptr(), _lib.stream_handle(dev))
+        _lib.check(rc, "agx_multipole_build_adjoint")
+        return gphi, gdphi, None
+
+
 class FittableMultipole(torch.nn.Module):
     """Multipole potential with its raw coefficient table as torch parameters.
 
@@ -153,10 +193,34 @@ class FittableMultipole(torch.nn.Module):
         prog = d["program"]
         self._program = torch.as_tensor(pack_program(prog).view(np.uint8), device=dev)
         self._n_groups, self._n_steps = len(prog.grp_m), len(prog.step_l)
+        self._cols32 = torch.as_tensor(np.asarray(d["columns"], np.int32), device=dev)
+        self._modes = ({"log": 1, "s2": 2}.get(d["inner_mode"], 0), int(bool(d["inner_reset"])),
+                       1 if d["outer_mode"] == "log" else 0)
+        self._device_build = dev.type == "cuda" and len(r) <= 128
 
     # ---- differentiable table build -------------------------------------------
+    def _source(self, phi_table: torch.Tensor, dphi_table: torch.Tensor):
+        d = self._d
+        s = _lib.agx_multipole_source()
+        s.phi, s.dphi, s.r = phi_table.data_ptr(), dphi_table.data_ptr(), self.r.data_ptr()
+        s.cols, s.Mf, s.Md = self._cols32.data_ptr(), self.Mf.data_ptr(), self.Md.data_ptr()
+        s.nR, s.n_cols, s.n_lm = phi_table.shape[0], phi_table.shape[1], d["n_lm"]
+        s.log_scaling = int(d["log_scaling"])
+        s.inner_mode, s.inner_reset, s.outer_mode = self._modes
+        s.dlogr = d["dlogr"]
+        return s
+
     def tables(self):
-        """(poly_km (n_int, n_lm, 6), scal [invPhi0, inner U, W, outer U, W])."""
+        """(poly_km (n_int, n_lm, 6), scal [invPhi0, inner U, W, outer U, W]):
+        on the GPU one agx_multipole_build launch (and one adjoint launch pair
+        in backward), elsewhere the same steps as torch operations."""
+        if self._device_build:
+            return _MultipoleBuild.apply(self.phi_table.contiguous(), self.dphi_table.contiguous(),
+                                         self)
+        return self.tables_torch()
+
+    def tables_torch(self):
+        """The rebuild as torch operations (any device)."""
         d = self._d
         r = self.r
         P = self.phi_table[:, self.cols]

@@ -199,6 +199,41 @@ int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xyz, int64_t 
                           void *stream);
 int64_t agx_multipole_adjoint_work_size(int64_t n);
 
+/* The raw coefficient table of a Multipole on the device and the frozen
+ * choices of its host build (multipole_prep.build_multipole_data, restating
+ * _build_multipole_data, gpu_potential.py:408-626): term columns in kernel
+ * order, law modes, and the C2-quintic system as two dense maps
+ * d2 = Mf f + Md df (nR x nR, row-major).  Used to rebuild poly_km and the law
+ * scalars on the device, differentiably. */
+typedef struct agx_multipole_source {
+    const double *phi;      /* [nR][n_cols] Phi_lm(r_i), the coefficient file's columns */
+    const double *dphi;     /* [nR][n_cols] dPhi_lm/dr(r_i) */
+    const double *r;        /* [nR] radii (log-uniform) */
+    const int32_t *cols;    /* [n_lm] file column of each kernel term, (0,0) first */
+    const double *Mf, *Md;  /* [nR][nR] */
+    int32_t nR, n_cols, n_lm;
+    int32_t log_scaling;
+    int32_t inner_mode;     /* 0 constant, 1 logarithmic, 2 s = 2 core */
+    int32_t inner_reset;    /* s = 2 core with invPhi0 = 0 */
+    int32_t outer_mode;     /* 0 Keplerian, 1 logarithmic */
+    int32_t pad;
+    double dlogr;
+} agx_multipole_source;
+
+/* poly_km [n_int][n_lm][6] and law [invPhi0, inner_U, inner_W, outer_U,
+ * outer_W] (the array agx_multipole.law points at) from the raw table, one
+ * workgroup per term; nR <= 128.  No reference counterpart: the device-side
+ * counterpart of the host build, for fitting loops that change the table
+ * every step. */
+int agx_multipole_build(const agx_multipole_source *src, double *d_poly, double *d_law,
+                        void *stream);
+/* Reverse of agx_multipole_build: ADDS dL/dphi, dL/ddphi [nR][n_cols] (the
+ * caller zeroes them) given dL/dpoly_km and dL/dlaw; d_work is one double of
+ * device scratch.  The (0,0) column's gradient is summed with FP64 atomics. */
+int agx_multipole_build_adjoint(const agx_multipole_source *src, const double *d_gpoly,
+                                const double *d_glaw, double *d_gphi, double *d_gdphi,
+                                double *d_work, void *stream);
+
 /* Reverse mode of agx_cylspline_eval's potential+force outputs.  Given
  * dL/dPhi (d_gphi, [N], may be NULL) and dL/dF (d_gforce, [N][3], may be
  * NULL) writes dL/dx into d_gxyz [N][3] (may be NULL; a Hessian-vector

@@ -209,3 +209,38 @@ def test_gradient_additive_over_batches():
     for full, summed in zip((gP, gD), acc):
         assert (torch.max(torch.abs(full - summed)) / torch.max(torch.abs(full))).item() < 1e-11
     assert torch.equal(gx, torch.cat(gx_parts))
+
+
+# (inner_mode, inner_reset, outer_mode, log_scaling) forced on a fixture: every
+# branch of the device rebuild and its adjoint against the torch rebuild
+MODES = [None, ("log", False, "log", 1), ("const", False, "kepler", 1),
+         ("s2", True, "kepler", 1), ("s2", False, "log", 0)]
+
+
+@pytest.mark.parametrize("modes", MODES, ids=["fixture", "log-log", "const-kepler", "s2reset-kepler",
+                                              "linear"])
+@pytest.mark.parametrize("name", ["fire_dense", "lmc", "halo_lmax8"])
+def test_device_build_matches_torch_rebuild(name, modes):
+    """agx_multipole_build / _adjoint equal the torch rebuild and its autograd
+    (poly and law to 1e-12 of each column's scale, gradients to 1e-11)."""
+    from agama_gpu_amd.autograd import FittableMultipole
+    c = _coefs(name)
+    print(f"case {name} modes={modes}", flush=True)
+    m = FittableMultipole(c, device=DEV)
+    assert m._device_build
+    if modes is not None:
+        im, reset, om, ls = modes
+        m._d = dict(m._d, inner_mode=im, inner_reset=reset, outer_mode=om, log_scaling=ls)
+        m._modes = ({"log": 1, "s2": 2}.get(im, 0), int(reset), 1 if om == "log" else 0)
+    poly_d, law_d = m.tables()
+    poly_t, law_t = m.tables_torch()
+    scale = poly_t.abs().amax(dim=(0, 2), keepdim=True).clamp_min(1e-300)
+    assert (torch.max(torch.abs(poly_d - poly_t) / scale)).item() < 1e-12
+    assert torch.allclose(law_d, law_t, rtol=1e-13, atol=0.0)
+    rng = np.random.default_rng(3)
+    Gp = torch.as_tensor(rng.normal(size=poly_t.shape), device=DEV)
+    Gl = torch.as_tensor(rng.normal(size=5), device=DEV)
+    gd = torch.autograd.grad((poly_d * Gp).sum() + (law_d * Gl).sum(), [m.phi_table, m.dphi_table])
+    gt = torch.autograd.grad((poly_t * Gp).sum() + (law_t * Gl).sum(), [m.phi_table, m.dphi_table])
+    for a, b in zip(gd, gt):
+        assert (torch.max(torch.abs(a - b)) / torch.max(torch.abs(b))).item() < 1e-11
