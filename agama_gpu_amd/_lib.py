@@ -88,9 +88,18 @@ BATCH_ITEM_DTYPE = np.dtype([("start", "<i8"), ("count", "<i4"), ("pot", "<i4"),
                              ("path", "<i4"), ("pad", "<i4")])
 
 
+_options = {}
+
+
 def set_option(key: int, value: int) -> None:
     """Process-wide tuning knob of the native library (see agx_set_option)."""
     check(load().agx_set_option(int(key), int(value)), "agx_set_option")
+    _options[int(key)] = int(value)
+
+
+def get_option(key: int, default: int = 0) -> int:
+    """The value last set through set_option (the library's default otherwise)."""
+    return _options.get(int(key), default)
 
 # (name, restype, argtypes) of every exported entry point declared in
 # include/agama_gpu.h
@@ -110,7 +119,10 @@ _SIGNATURES = {
     "agx_multipole_adjoint": (C.c_int, [C.POINTER(agx_multipole), C.c_void_p, C.c_int64,
                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
-    "agx_multipole_adjoint_work_size": (C.c_int64, [C.c_int64]),
+    "agx_multipole_work_size": (C.c_int64, [C.c_int64]),
+    "agx_multipole_eval_sorted": (C.c_int, [C.POINTER(agx_multipole), C.c_void_p, C.c_int64,
+                                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                            C.c_void_p, C.c_int64, C.c_void_p]),
     "agx_multipole_build": (C.c_int, [C.POINTER(agx_multipole_source), C.c_void_p, C.c_void_p,
                                       C.c_void_p]),
     "agx_multipole_build_adjoint": (C.c_int, [C.POINTER(agx_multipole_source), C.c_void_p,

@@ -36,6 +36,7 @@ from .base import norm_device
 from .coefs import CylSplineCoefs, MultipoleCoefs
 from .composite import _linear_extrapolating_spline
 from .cylspline_prep import MUL0, MUL1, build_cylspline_data, ptilde
+from .multipole import eval_table
 from .multipole_prep import build_multipole_data, pack_program
 
 F64 = torch.float64
@@ -94,10 +95,8 @@ class _MultipoleEval(torch.autograd.Function):
         n = xyz.shape[0]
         phi = torch.empty(n, dtype=F64, device=xyz.device)
         force = torch.empty((n, 3), dtype=F64, device=xyz.device)
-        rc = _lib.load().agx_multipole_eval(C.byref(desc), xyz.data_ptr(), n, phi.data_ptr(),
-                                            force.data_ptr(), None, _lib.AGX_PHI | _lib.AGX_FORCE,
-                                            _lib.stream_handle(xyz.device))
-        _lib.check(rc, "agx_multipole_eval")
+        eval_table(desc, owner._d, xyz, phi, force, None, _lib.AGX_PHI | _lib.AGX_FORCE,
+                   xyz.device)
         ctx.save_for_backward(xyz, poly, scal)
         ctx.owner = owner
         return phi, force
@@ -114,7 +113,7 @@ class _MultipoleEval(torch.autograd.Function):
         gphi = None if gphi is None else gphi.contiguous()
         gforce = None if gforce is None else gforce.contiguous()
         L = _lib.load()
-        nwork = int(L.agx_multipole_adjoint_work_size(n))
+        nwork = int(L.agx_multipole_work_size(n))
         work = torch.empty(nwork, dtype=torch.uint8, device=xyz.device)
         rc = L.agx_multipole_adjoint(
             C.byref(desc), xyz.data_ptr(), n, None if gphi is None else gphi.data_ptr(),

@@ -24,17 +24,6 @@ constexpr int kAdjBlock = 256;
 // LDS slabs are used while 4 of them (one per wavefront) fit in this many bytes
 constexpr int kAdjSlabLds = 53 * 1024;
 
-// Sort key: 0 outside the grid (origin, inner and outer laws), k + 1 in interval k.
-struct MpBinKey {
-    MPK P;
-    const double *xyz;
-    __device__ int operator()(int64_t i) const
-    {
-        const Where w = classify(P, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
-        return (w.kind == 2) ? w.k + 1 : 0;
-    }
-};
-
 // s^j a + j s^(j-1) b for j = 0..5, summed over the wavefront and added to
 // g[0..5]: a wavefront-private LDS slab (SLAB) or the table gradient in memory.
 template <bool SLAB>
@@ -264,11 +253,6 @@ __global__ __launch_bounds__(kAdjBlock, 3) void mp_adjoint_kernel(
 }  // namespace agx
 
 using namespace agx;
-
-extern "C" int64_t agx_multipole_adjoint_work_size(int64_t n)
-{
-    return n < 0 ? 0 : sort_work_size(n, kMaxBins);
-}
 
 extern "C" int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xyz, int64_t n,
                                      const double *d_gphi, const double *d_gforce, double *d_gxyz,

@@ -1,5 +1,6 @@
 // Multipole kernels and C-ABI entry points (gfx950, FP64).
 #include "multipole_dev.h"
+#include "sort_dev.h"
 
 namespace agx {
 
@@ -85,6 +86,35 @@ __global__ __launch_bounds__(BLOCK) void mp_lds_kernel(MPK P, const double *__re
             extrap<ORD>(P, w, x, y, z, o, inv4piG);
         store<ORD>(o, i, phi, force, deriv);
     }
+}
+
+// The interval-sorted order (sort_dev.h): 64 particles of one interval per
+// wavefront and chunk, coefficients by scalar loads as in the binned kernel,
+// but every wavefront full except one per interval.
+template <int ORD>
+__global__ __launch_bounds__(kBlock) void mp_sorted_kernel(MPK P, const double *__restrict__ xyz,
+                                                           double *__restrict__ phi,
+                                                           double *__restrict__ force,
+                                                           double *__restrict__ deriv,
+                                                           double inv4piG, SortWork W, int run_len)
+{
+    P = with_law(P);
+    for_each_sorted_chunk(W, P.n_int + 1, run_len, [&](int b, bool valid, int64_t i) {
+        if (!valid) return;
+        const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+        PointOut o{};
+        if (b > 0) {
+            const int k = b - 1;
+            const double r2 = fma(x, x, fma(y, y, z * z));
+            const double r = sqrt(r2);
+            interior<ORD, true>(P, P.poly + (int64_t)k * P.n_lm * 6, x, y, z, r2, r, agx_log(r), k, o,
+                                inv4piG);
+        } else {
+            const Where w = classify(P, x, y, z);
+            extrap<ORD>(P, w, x, y, z, o, inv4piG);
+        }
+        store<ORD>(o, i, phi, force, deriv);
+    });
 }
 
 static int check_mp(const agx_multipole *mp, const double *xyz, int64_t n)
@@ -174,7 +204,7 @@ static void launch_mp(const agx_multipole *mp, const MPK &P, const double *xyz, 
     int path = g_mp_path;
     // automatic: the LDS-staged table whenever it fits (and N is not tiny),
     // else binning for many terms and large N, else per-lane table reads
-    if (path == 0) path = (lds_ok && n >= 4096) ? 3 : (binned_auto(mp, n) ? 2 : 1);
+    if (path == 0 || path == 4) path = (lds_ok && n >= 4096) ? 3 : (binned_auto(mp, n) ? 2 : 1);
     if (path == 3 && !lds_ok) path = 1;
     if (path == 2 && mp->n_int + 1 > kMaxBins) path = 1;
     if (path == 3) {
@@ -223,7 +253,7 @@ extern "C" int agx_set_option(int key, int64_t value)
 {
     switch (key) {
     case AGX_OPT_MP_PATH:
-        if (value < 0 || value > 3) return AGX_E_SIZE;
+        if (value < 0 || value > 4) return AGX_E_SIZE;
         g_mp_path = (int)value;
         return AGX_OK;
     case AGX_OPT_MP_TILE:
@@ -277,6 +307,50 @@ extern "C" int agx_multipole_eval(const agx_multipole *mp, const double *d_xyz, 
         launch_mp<ORD_GRAD>(mp, P, d_xyz, n, phi, force, nullptr, 0.0, s);
     else
         launch_mp<ORD_POT>(mp, P, d_xyz, n, phi, nullptr, nullptr, 0.0, s);
+    return hipGetLastError() == hipSuccess ? AGX_OK : AGX_E_LAUNCH;
+}
+
+extern "C" int64_t agx_multipole_work_size(int64_t n)
+{
+    return n < 0 ? 0 : sort_work_size(n, kMaxBins);
+}
+
+extern "C" int agx_multipole_eval_sorted(const agx_multipole *mp, const double *d_xyz, int64_t n,
+                                         double *d_phi, double *d_force, double *d_deriv,
+                                         uint32_t what, void *d_work, int64_t work_bytes,
+                                         void *stream)
+{
+    int rc = check_mp(mp, d_xyz, n);
+    if (rc) return rc;
+    if (what == 0 || (what & ~(AGX_PHI | AGX_FORCE | AGX_DERIV))) return AGX_E_WHAT;
+    if (n > (int64_t)0x7FFFFFFF - 64 * kMaxBins || mp->n_int + 1 > kMaxBins) return AGX_E_SIZE;
+    if (n == 0) return AGX_OK;
+    if (((what & AGX_PHI) && !d_phi) || ((what & AGX_FORCE) && !d_force) ||
+        ((what & AGX_DERIV) && !d_deriv))
+        return AGX_E_NULL;
+    if (!d_work) return AGX_E_NULL;
+    if (work_bytes < sort_work_size(n, kMaxBins)) return AGX_E_WORKSPACE;
+    MPK P = make_args(mp);
+    hipStream_t s = (hipStream_t)stream;
+    const SortWork W = sort_work(d_work, n, kMaxBins);
+    const int nb = mp->n_int + 1;
+    sort_launch(MpBinKey{P, d_xyz}, n, nb, W, s);
+    double *phi = (what & AGX_PHI) ? d_phi : nullptr;
+    double *force = (what & AGX_FORCE) ? d_force : nullptr;
+    double *deriv = (what & AGX_DERIV) ? d_deriv : nullptr;
+    const int run_len = sort_run_len(n, nb, 256LL * 12);
+    const int64_t chunks = (n + 63) / 64 + nb;
+    const int64_t wg_need = (chunks + run_len - 1) / run_len / (kBlock / 64) + 1;
+    const unsigned grid = (unsigned)(wg_need < 2048 ? wg_need : 2048);
+    if (deriv)
+        hipLaunchKernelGGL(mp_sorted_kernel<ORD_HESS>, dim3(grid), dim3(kBlock), 0, s, P, d_xyz, phi,
+                           force, deriv, 0.0, W, run_len);
+    else if (force)
+        hipLaunchKernelGGL(mp_sorted_kernel<ORD_GRAD>, dim3(grid), dim3(kBlock), 0, s, P, d_xyz, phi,
+                           force, nullptr, 0.0, W, run_len);
+    else
+        hipLaunchKernelGGL(mp_sorted_kernel<ORD_POT>, dim3(grid), dim3(kBlock), 0, s, P, d_xyz, phi,
+                           nullptr, nullptr, 0.0, W, run_len);
     return hipGetLastError() == hipSuccess ? AGX_OK : AGX_E_LAUNCH;
 }
 

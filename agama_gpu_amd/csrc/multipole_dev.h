@@ -503,6 +503,18 @@ __device__ __forceinline__ void binned_tiles(const MPK &P, const double *__restr
     }
 }
 
+// Sort key of the interval-sorted traversals (sort_dev.h): 0 outside the grid
+// (origin, inner and outer laws), k + 1 in interval k.
+struct MpBinKey {
+    MPK P;
+    const double *xyz;
+    __device__ int operator()(int64_t i) const
+    {
+        const Where w = classify(P, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+        return (w.kind == 2) ? w.k + 1 : 0;
+    }
+};
+
 inline MPK make_args(const agx_multipole *mp)
 {
     MPK P;

@@ -23,6 +23,36 @@ from .coefs import MultipoleCoefs, read_coefs
 from .multipole_prep import INV_4PIG, build_multipole_data, pack_program
 
 
+# Tables above this size do not fit the LDS-staged path; from this many points
+# on they are evaluated through the interval-sorted path.
+LDS_TABLE_BYTES = 160 * 1024 - 256
+SORTED_MIN_N = 200_000
+
+
+def eval_table(desc, data: dict, xyz, phi, force, deriv, what: int, device):
+    """One evaluation of a Multipole descriptor: agx_multipole_eval, or for
+    tables too large for LDS at large N (and whenever AGX_OPT_MP_PATH is 4)
+    agx_multipole_eval_sorted with a workspace from torch's allocator."""
+    n = xyz.shape[0]
+    L = _lib.load()
+    opt = _lib.get_option(_lib.AGX_OPT_MP_PATH)
+    table = data["n_intervals"] * data["n_lm"] * 48
+    sorted_ok = data["n_intervals"] + 1 <= 128 and n > 0
+    use_sorted = sorted_ok and (opt == 4 or (opt == 0 and table > LDS_TABLE_BYTES and
+                                               n >= SORTED_MIN_N))
+    stream = _lib.stream_handle(device)
+    if use_sorted:
+        nwork = int(L.agx_multipole_work_size(n))
+        work = torch.empty(nwork, dtype=torch.uint8, device=device)
+        rc = L.agx_multipole_eval_sorted(C.byref(desc), xyz.data_ptr(), n, ptr(phi), ptr(force),
+                                         ptr(deriv), what, work.data_ptr(), nwork, stream)
+        _lib.check(rc, "agx_multipole_eval_sorted")
+    else:
+        rc = L.agx_multipole_eval(C.byref(desc), xyz.data_ptr(), n, ptr(phi), ptr(force),
+                                  ptr(deriv), what, stream)
+        _lib.check(rc, "agx_multipole_eval")
+
+
 class MultipolePotentialGPU(PotentialBase):
     """GPU evaluator for an Agama Multipole expansion (lmax <= 32)."""
 
@@ -87,10 +117,7 @@ class MultipolePotentialGPU(PotentialBase):
         phi, force, deriv = alloc_outputs(n, self.device, want_phi, want_force, want_deriv, out)
         what = (_lib.AGX_PHI if want_phi else 0) | (_lib.AGX_FORCE if want_force else 0) | \
                (_lib.AGX_DERIV if want_deriv else 0)
-        L = _lib.load()
-        rc = L.agx_multipole_eval(C.byref(self._desc), xyz.data_ptr(), n, ptr(phi), ptr(force),
-                                  ptr(deriv), what, _lib.stream_handle(self.device))
-        _lib.check(rc, "agx_multipole_eval")
+        eval_table(self._desc, self._data, xyz, phi, force, deriv, what, self.device)
         return phi, force, deriv
 
     def _density(self, xyz, t=0.0):

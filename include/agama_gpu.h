@@ -91,6 +91,16 @@ int agx_multipole_eval(const agx_multipole *mp, const double *d_xyz, int64_t n,
                        double *d_phi, double *d_force, double *d_deriv,
                        uint32_t what, void *stream);
 
+/* As agx_multipole_eval, through the interval-sorted path: the points are
+ * first sorted by radial interval into d_work (agx_multipole_work_size(N)
+ * bytes, overwritten, no clearing needed), then every wavefront evaluates 64
+ * points of one interval with the interval's coefficients in scalar
+ * registers.  For tables too large for LDS and large N.  Needs n_int <= 127
+ * and N < 2^31 - 8192. */
+int agx_multipole_eval_sorted(const agx_multipole *mp, const double *d_xyz, int64_t n,
+                              double *d_phi, double *d_force, double *d_deriv, uint32_t what,
+                              void *d_work, int64_t work_bytes, void *stream);
+
 /* Density rho = Laplacian(Phi) / (4 pi G) at N points.
  * Replaces multipole_density_kernel (_multipole_potential_kernel.cu:809-958)
  * as launched by _launch_density (gpu_potential.py:784-793). */
@@ -186,7 +196,7 @@ int agx_multipole_pack(const agx_multipole *mp, void *dst, int64_t size);
  * product) and ADDS dL/d(poly_km) into d_gpoly [n_int][n_lm][6] and
  * dL/d(invPhi0, inner_U, inner_W, outer_U, outer_W) into d_gscal [5]; the
  * caller zeroes those first.  d_work is device scratch of at least
- * agx_multipole_adjoint_work_size(N) bytes (about 10 N; the particles are
+ * agx_multipole_work_size(N) bytes (about 10 N; the particles are
  * sorted by radial interval there); it is overwritten and needs no clearing.
  * Table gradients are summed per wavefront and interval before FP64 atomic
  * adds, so their last bits depend on the order of the atomics.  Needs
@@ -197,7 +207,8 @@ int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xyz, int64_t 
                           const double *d_gphi, const double *d_gforce, double *d_gxyz,
                           double *d_gpoly, double *d_gscal, void *d_work, int64_t work_bytes,
                           void *stream);
-int64_t agx_multipole_adjoint_work_size(int64_t n);
+/* Device scratch (bytes) of the interval-sorted entry points for N points. */
+int64_t agx_multipole_work_size(int64_t n);
 
 /* The raw coefficient table of a Multipole on the device and the frozen
  * choices of its host build (multipole_prep.build_multipole_data, restating
@@ -331,7 +342,10 @@ int agx_leapfrog(const agx_component *d_leaves, int32_t n_leaves,
  *                        (wave-uniform interval, scalar table reads; needs
  *                        n_int <= 127), 3 one particle per lane with the
  *                        table staged in LDS (tables up to 160 KiB; larger
- *                        ones take path 1)
+ *                        ones take path 1), 4 interval-sorted where the caller
+ *                        gives a workspace (agx_multipole_eval_sorted; the
+ *                        Python layer then always takes it), automatic in
+ *                        agx_multipole_eval
  *   AGX_OPT_MP_TILE      particles per binned tile (0 = automatic, <= 8192)
  *   AGX_OPT_MP_BIN_MIN_N smallest N for which the automatic path bins
  *   AGX_OPT_MP_BIN_MIN_TERMS smallest number of active harmonics for which

@@ -62,8 +62,8 @@ def test_adjoint_workspace_validation_without_gpu():
     from agama_gpu_amd import _lib
     L = _lib.load()
     n = 1000
-    need = L.agx_multipole_adjoint_work_size(n)
-    assert need >= 9 * n and L.agx_multipole_adjoint_work_size(-1) == 0
+    need = L.agx_multipole_work_size(n)
+    assert need >= 9 * n and L.agx_multipole_work_size(-1) == 0
     d = _lib.agx_multipole()
     d.poly_km = d.program = 8
     d.n_int, d.n_lm, d.lmax = 4, 1, 0

@@ -46,7 +46,7 @@ def points(n, seed=0):
     return x
 
 
-@pytest.fixture(params=[1, 2, 3], ids=["lane", "binned", "lds"])
+@pytest.fixture(params=[1, 2, 3, 4], ids=["lane", "binned", "lds", "sorted"])
 def mp_path(request):
     """Force one evaluation path of the native library for the test."""
     from agama_gpu_amd import _lib
