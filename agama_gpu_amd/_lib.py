@@ -76,6 +76,7 @@ AGX_OPT_MP_PATH, AGX_OPT_MP_TILE, AGX_OPT_MP_BIN_MIN_N, AGX_OPT_MP_BIN_MIN_TERMS
 AGX_OPT_MP_LDS_GRID = 6
 AGX_OPT_MP_BIN_GRID = 7
 AGX_OPT_GRID_PER_CU = 8
+AGX_OPT_SORT_GRID = 9
 AGX_BATCH_MAX_ITEM = 16384
 AGX_BATCH_LANE, AGX_BATCH_LDS = 0, 1
 

@@ -280,6 +280,10 @@ extern "C" int agx_set_option(int key, int64_t value)
         if (value < 1 || value > 64) return AGX_E_SIZE;
         g_mp_lds_grid = (int)value;
         return AGX_OK;
+    case AGX_OPT_SORT_GRID:
+        if (value < 0 || value > 4096) return AGX_E_SIZE;
+        g_sort_grid = (int)value;
+        return AGX_OK;
     default:
         return AGX_E_WHAT;
     }

@@ -129,12 +129,21 @@ __global__ __launch_bounds__(kSortBlock) void sort_scatter_kernel(int64_t n, int
 
 }  // namespace
 
+// Workgroups of the count and scatter launches: every workgroup adds its
+// histogram to the global one with one atomic per nonzero bin, and those
+// atomics serialise per bin, so the grid grows with N only slowly: about one
+// workgroup per 4096 points, 64 to 1024 (measured on MI355X: 256 best at
+// N = 1M, 1024 at N = 10M).  0 = that rule; agx_set_option AGX_OPT_SORT_GRID.
+inline int g_sort_grid = 0;
+
 // The four sort launches on `stream`; key is a device functor int(int64_t i).
 template <class Key>
 inline void sort_launch(Key key, int64_t n, int nb, const SortWork &W, hipStream_t stream)
 {
     const int64_t n_blk = (n + kSortBlock - 1) / kSortBlock;
-    const unsigned grid = (unsigned)(n_blk < 1024 ? n_blk : 1024);
+    int64_t want = g_sort_grid;
+    if (want <= 0) want = n / 4096 < 64 ? 64 : (n / 4096 > 1024 ? 1024 : n / 4096);
+    const unsigned grid = (unsigned)(n_blk < want ? n_blk : want);
     hipLaunchKernelGGL(sort_init_kernel, dim3(1), dim3(kSortBlock), 0, stream, W, nb);
     hipLaunchKernelGGL(sort_count_kernel<Key>, dim3(grid), dim3(kSortBlock), (size_t)nb * 4, stream,
                        key, n, W, nb);

@@ -363,6 +363,9 @@ int agx_leapfrog(const agx_component *d_leaves, int32_t n_leaves,
 #define AGX_OPT_GRID_PER_CU 8 /* workgroups per CU of the grid-stride kernels (lane
                                  Multipole, CylSpline, composite, leapfrog;
                                  1..64, default 32) */
+#define AGX_OPT_SORT_GRID 9   /* workgroups of the interval / cell sort's count and
+                                 scatter launches (1..4096; default 0: about one
+                                 per 4096 points, 64 to 1024) */
 int agx_set_option(int key, int64_t value);
 
 /* Library identification and error text. */
