@@ -173,3 +173,31 @@ def test_gradient_repeatable_and_force_only():
     gF = torch.autograd.grad((wF * m(xyz)[1]).sum(), m.phi_table)[0]
     gF0 = torch.autograd.grad(_loss(m(xyz), torch.zeros_like(wphi), wF), m.phi_table)[0]
     assert (torch.max(torch.abs(gF - gF0)) / torch.max(torch.abs(gF0))).item() < 1e-12
+
+
+def test_gradient_additive_over_batches():
+    """Large N (runs of several chunks per wavefront, many cells): the table
+    gradient of the whole set equals the sum over three unequal batches, and
+    dL/dx matches row by row."""
+    from agama_gpu_amd.autograd import FittableCylSpline
+    n = 600_000
+    x = _points("all", n=n, seed=9)
+    print(f"case bar n={n}", flush=True)
+    m = FittableCylSpline(_coefs(), device=DEV)
+    xyz = torch.as_tensor(x, device=DEV)
+    wphi, wF = _weights(n, 10)
+
+    def grads(sl):
+        xs = xyz[sl].clone().requires_grad_(True)
+        out = m(xs)
+        return torch.autograd.grad(_loss(out, wphi[sl], wF[sl]), [m.phi_table, xs])
+
+    g, gx = grads(slice(0, n))
+    acc = torch.zeros_like(g)
+    gx_parts = []
+    for sl in (slice(0, 100), slice(100, 250_000), slice(250_000, n)):
+        a, gxs = grads(sl)
+        acc += a
+        gx_parts.append(gxs)
+    assert (torch.max(torch.abs(g - acc)) / torch.max(torch.abs(g))).item() < 1e-11
+    assert torch.equal(gx, torch.cat(gx_parts))
