@@ -72,6 +72,7 @@ AGX_MAX_COMPONENTS = 16
 AGX_T_MULTIPOLE, AGX_T_CYLSPLINE = 16, 17
 
 AGX_PHI, AGX_FORCE, AGX_DERIV = 1, 2, 4
+AGX_WORK_SORTED = 1
 AGX_OPT_MP_PATH, AGX_OPT_MP_TILE, AGX_OPT_MP_BIN_MIN_N, AGX_OPT_MP_BIN_MIN_TERMS = 1, 2, 3, 4
 AGX_OPT_MP_LDS_GRID = 6
 AGX_OPT_MP_BIN_GRID = 7
@@ -119,7 +120,8 @@ _SIGNATURES = {
                                         C.c_int64, C.c_void_p, C.c_void_p]),
     "agx_multipole_adjoint": (C.c_int, [C.POINTER(agx_multipole), C.c_void_p, C.c_int64,
                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                        C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+                                        C.c_void_p, C.c_void_p, C.c_int64, C.c_uint32,
+                                        C.c_void_p]),
     "agx_multipole_work_size": (C.c_int64, [C.c_int64]),
     "agx_multipole_eval_sorted": (C.c_int, [C.POINTER(agx_multipole), C.c_void_p, C.c_int64,
                                             C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,

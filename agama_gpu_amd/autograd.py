@@ -95,8 +95,10 @@ class _MultipoleEval(torch.autograd.Function):
         n = xyz.shape[0]
         phi = torch.empty(n, dtype=F64, device=xyz.device)
         force = torch.empty((n, 3), dtype=F64, device=xyz.device)
-        eval_table(desc, owner._d, xyz, phi, force, None, _lib.AGX_PHI | _lib.AGX_FORCE,
-                   xyz.device)
+        # the sorted path leaves the interval sort of xyz in its workspace:
+        # the backward traverses the same order
+        ctx.work = eval_table(desc, owner._d, xyz, phi, force, None,
+                              _lib.AGX_PHI | _lib.AGX_FORCE, xyz.device)
         ctx.save_for_backward(xyz, poly, scal)
         ctx.owner = owner
         return phi, force
@@ -114,11 +116,14 @@ class _MultipoleEval(torch.autograd.Function):
         gforce = None if gforce is None else gforce.contiguous()
         L = _lib.load()
         nwork = int(L.agx_multipole_work_size(n))
-        work = torch.empty(nwork, dtype=torch.uint8, device=xyz.device)
+        work, flags = ctx.work, _lib.AGX_WORK_SORTED
+        ctx.work = None
+        if work is None:
+            work, flags = torch.empty(nwork, dtype=torch.uint8, device=xyz.device), 0
         rc = L.agx_multipole_adjoint(
             C.byref(desc), xyz.data_ptr(), n, None if gphi is None else gphi.data_ptr(),
             None if gforce is None else gforce.data_ptr(), None if gx is None else gx.data_ptr(),
-            gpoly.data_ptr(), gscal.data_ptr(), work.data_ptr(), nwork,
+            gpoly.data_ptr(), gscal.data_ptr(), work.data_ptr(), nwork, flags,
             _lib.stream_handle(xyz.device))
         _lib.check(rc, "agx_multipole_adjoint")
         return gx, gpoly, gscal, None

@@ -257,7 +257,7 @@ using namespace agx;
 extern "C" int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xyz, int64_t n,
                                      const double *d_gphi, const double *d_gforce, double *d_gxyz,
                                      double *d_gpoly, double *d_gscal, void *d_work,
-                                     int64_t work_bytes, void *stream)
+                                     int64_t work_bytes, uint32_t flags, void *stream)
 {
     if (!mp) return AGX_E_NULL;
     if (n < 0 || n > (int64_t)0x7FFFFFFF - 64 * kMaxBins) return AGX_E_SIZE;
@@ -271,7 +271,10 @@ extern "C" int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xy
     const SortWork W = sort_work(d_work, n, kMaxBins);
     const int nb = mp->n_int + 1;
     hipStream_t st = (hipStream_t)stream;
-    sort_launch(MpBinKey{P, d_xyz}, n, nb, W, st);
+    if (flags & AGX_WORK_SORTED)
+        sort_requeue(W, st);
+    else
+        sort_launch(MpBinKey{P, d_xyz}, n, nb, W, st);
     // about two runs per resident wavefront (3 per SIMD)
     const int run_len = sort_run_len(n, nb, 256LL * 12);
     const int64_t chunks = (n + 63) / 64 + nb;

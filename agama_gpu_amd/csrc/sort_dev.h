@@ -127,7 +127,19 @@ __global__ __launch_bounds__(kSortBlock) void sort_scatter_kernel(int64_t n, int
     }
 }
 
+__global__ void sort_requeue_kernel(SortWork W)
+{
+    if (threadIdx.x == 0) W.queue[0] = 0;
+}
+
 }  // namespace
+
+// Rewind the run queue of a workspace that already holds a sort (to traverse
+// the same sorted order again).
+inline void sort_requeue(const SortWork &W, hipStream_t stream)
+{
+    hipLaunchKernelGGL(sort_requeue_kernel, dim3(1), dim3(64), 0, stream, W);
+}
 
 // Workgroups of the count and scatter launches: every workgroup adds its
 // histogram to the global one with one atomic per nonzero bin, and those

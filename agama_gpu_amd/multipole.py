@@ -32,7 +32,8 @@ SORTED_MIN_N = 200_000
 def eval_table(desc, data: dict, xyz, phi, force, deriv, what: int, device):
     """One evaluation of a Multipole descriptor: agx_multipole_eval, or for
     tables too large for LDS at large N (and whenever AGX_OPT_MP_PATH is 4)
-    agx_multipole_eval_sorted with a workspace from torch's allocator."""
+    agx_multipole_eval_sorted with a workspace from torch's allocator.
+    Returns that workspace (it holds the interval sort of xyz) or None."""
     n = xyz.shape[0]
     L = _lib.load()
     opt = _lib.get_option(_lib.AGX_OPT_MP_PATH)
@@ -47,10 +48,11 @@ def eval_table(desc, data: dict, xyz, phi, force, deriv, what: int, device):
         rc = L.agx_multipole_eval_sorted(C.byref(desc), xyz.data_ptr(), n, ptr(phi), ptr(force),
                                          ptr(deriv), what, work.data_ptr(), nwork, stream)
         _lib.check(rc, "agx_multipole_eval_sorted")
-    else:
-        rc = L.agx_multipole_eval(C.byref(desc), xyz.data_ptr(), n, ptr(phi), ptr(force),
-                                  ptr(deriv), what, stream)
-        _lib.check(rc, "agx_multipole_eval")
+        return work
+    rc = L.agx_multipole_eval(C.byref(desc), xyz.data_ptr(), n, ptr(phi), ptr(force),
+                              ptr(deriv), what, stream)
+    _lib.check(rc, "agx_multipole_eval")
+    return None
 
 
 class MultipolePotentialGPU(PotentialBase):

@@ -198,6 +198,9 @@ int agx_multipole_pack(const agx_multipole *mp, void *dst, int64_t size);
  * caller zeroes those first.  d_work is device scratch of at least
  * agx_multipole_work_size(N) bytes (about 10 N; the particles are
  * sorted by radial interval there); it is overwritten and needs no clearing.
+ * flags: AGX_WORK_SORTED if d_work already holds the sort of these d_xyz
+ * for this radial grid (left by agx_multipole_eval_sorted or an earlier
+ * adjoint call with the same points and grid): the sort is skipped.
  * Table gradients are summed per wavefront and interval before FP64 atomic
  * adds, so their last bits depend on the order of the atomics.  Needs
  * n_int <= 127 and N < 2^31 - 8192.  No reference counterpart (the
@@ -206,7 +209,8 @@ int agx_multipole_pack(const agx_multipole *mp, void *dst, int64_t size);
 int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xyz, int64_t n,
                           const double *d_gphi, const double *d_gforce, double *d_gxyz,
                           double *d_gpoly, double *d_gscal, void *d_work, int64_t work_bytes,
-                          void *stream);
+                          uint32_t flags, void *stream);
+#define AGX_WORK_SORTED 1u
 /* Device scratch (bytes) of the interval-sorted entry points for N points. */
 int64_t agx_multipole_work_size(int64_t n);
 

@@ -244,3 +244,29 @@ def test_device_build_matches_torch_rebuild(name, modes):
     gt = torch.autograd.grad((poly_t * Gp).sum() + (law_t * Gl).sum(), [m.phi_table, m.dphi_table])
     for a, b in zip(gd, gt):
         assert (torch.max(torch.abs(a - b)) / torch.max(torch.abs(b))).item() < 1e-11
+
+
+def test_backward_reuses_forward_sort():
+    """With the interval-sorted forward (AGX_OPT_MP_PATH 4) the backward
+    traverses the forward's sort (AGX_WORK_SORTED) instead of sorting again:
+    same gradients as with an LDS-path forward, to round-off."""
+    from agama_gpu_amd import _lib
+    from agama_gpu_amd.autograd import FittableMultipole
+    c = _coefs("halo_lmax8")
+    x = _points(c, n=50_000, seed=4)
+    m = FittableMultipole(c, device=DEV)
+    rng = np.random.default_rng(6)
+    xyz = torch.as_tensor(x, device=DEV)
+    wphi = torch.as_tensor(rng.normal(size=len(x)), device=DEV)
+    wF = torch.as_tensor(rng.normal(size=(len(x), 3)), device=DEV)
+    out = []
+    try:
+        for path in (0, 4):
+            _lib.set_option(_lib.AGX_OPT_MP_PATH, path)
+            xs = xyz.clone().requires_grad_(True)
+            out.append(torch.autograd.grad(_loss(m, xs, wphi, wF),
+                                           [m.phi_table, m.dphi_table, xs]))
+    finally:
+        _lib.set_option(_lib.AGX_OPT_MP_PATH, 0)
+    for a, b in zip(*out):
+        assert (torch.max(torch.abs(a - b)) / torch.max(torch.abs(a))).item() < 1e-12

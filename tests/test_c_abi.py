@@ -68,7 +68,7 @@ def test_adjoint_workspace_validation_without_gpu():
     d.poly_km = d.program = 8
     d.n_int, d.n_lm, d.lmax = 4, 1, 0
     args = (C.byref(d), 8, n, 8, None, None, 8, 8)
-    assert L.agx_multipole_adjoint(*args, None, need, None) == -1
-    assert L.agx_multipole_adjoint(*args, 8, need - 1, None) == -7
+    assert L.agx_multipole_adjoint(*args, None, need, 0, None) == -1
+    assert L.agx_multipole_adjoint(*args, 8, need - 1, 0, None) == -7
     d.n_int = 128                       # n_int + 1 bins > 128
-    assert L.agx_multipole_adjoint(*args, 8, need, None) == -2
+    assert L.agx_multipole_adjoint(*args, 8, need, 0, None) == -2
