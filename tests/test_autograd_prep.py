@@ -73,13 +73,17 @@ def test_spline_center_weights():
         assert np.allclose(g[:, [0, 2]], 0.0)
 
 
-def test_cylspline_tables_match_numpy_build():
+@pytest.mark.parametrize("name", ["bar", "disk"])
+def test_cylspline_tables_match_numpy_build(name):
     """The CylSpline rebuild (mirroring, log-scaling, spline slopes by matrix,
-    outer fit by pseudo-inverse) reproduces cylspline_prep's node data and W."""
+    outer fit by pseudo-inverse) reproduces cylspline_prep's node data and W:
+    the reference's full-z bar fixture and the half-space disk table."""
     from agama_gpu_amd.autograd import FittableCylSpline
     from agama_gpu_amd.coefs import read_coefs
     from agama_gpu_amd.cylspline_prep import build_cylspline_data
-    c = read_coefs(FIXTURES / "600.bar.none_8.coef_cylsp_DR")
+    from agama_gpu_amd.workloads import mw_disk_bar_coefs
+    c = read_coefs(FIXTURES / "600.bar.none_8.coef_cylsp_DR") if name == "bar" else \
+        mw_disk_bar_coefs()
     d = build_cylspline_data(c)
     m = FittableCylSpline(c, device="cpu")
     with torch.no_grad():

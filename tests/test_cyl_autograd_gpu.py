@@ -1,5 +1,7 @@
 """Gradients of the CylSpline adjoint kernel (agx_cylspline_adjoint) on gfx950,
-on the reference's bar fixture (17 harmonics, 25 x 49 grid, log-scaled m=0).
+on the reference's bar fixture (17 harmonics, 25 x 49 grid, log-scaled m=0)
+and, where marked, the Milky-Way disk + bar table (7 harmonics, a 32 x 32
+half-space grid mirrored to z < 0: the mirroring of the rebuild).
 
 * forward of the differentiable path = CylSplinePotentialGPU (1e-12 / 1e-11).
 * dL/dxyz is a Hessian-vector product: against the forward kernel's own
@@ -24,15 +26,17 @@ DEV = "cuda"
 FILE = FIXTURES / "600.bar.none_8.coef_cylsp_DR"
 
 
-def _coefs():
+def _coefs(name="bar"):
     from agama_gpu_amd.coefs import read_coefs
-    return read_coefs(FILE)
+    from agama_gpu_amd.workloads import mw_disk_bar_coefs
+    return read_coefs(FILE) if name == "bar" else mw_disk_bar_coefs()
 
 
-def _points(region, n=4096, seed=0):
-    """The grid spans R <= 500, |z| <= 500."""
+def _points(region, n=4096, seed=0, ext=500.0):
+    """The grid spans R <= ext, |z| <= ext (500 for the bar, 200 for the disk)."""
     rng = np.random.default_rng(seed)
-    lo, hi = {"grid": (0.05, 450.0), "outer": (720.0, 5000.0), "all": (0.05, 3000.0)}[region]
+    lo, hi = {"grid": (0.05, 0.9 * ext), "outer": (1.44 * ext, 10.0 * ext),
+              "all": (0.05, 6.0 * ext)}[region]
     r = np.exp(rng.uniform(np.log(lo), np.log(hi), n))
     ct = rng.uniform(-1, 1, n)
     ph = rng.uniform(0, 2 * np.pi, n)
@@ -41,8 +45,11 @@ def _points(region, n=4096, seed=0):
     if region == "grid":
         x[:3] = [[0.0, 0.0, 0.0], [0.0, 0.0, 3.0], [2.0, 0.5, -0.5]]
     elif region == "outer":
-        x[:2] = [[0.0, 0.0, 800.0], [900.0, 0.0, 0.0]]
+        x[:2] = [[0.0, 0.0, 1.6 * ext], [1.8 * ext, 0.0, 0.0]]
     return x
+
+
+EXT = {"bar": 500.0, "disk": 200.0}
 
 
 def _weights(n, seed):
@@ -55,13 +62,14 @@ def _loss(out, wphi, wF):
     return (wphi * out[0]).sum() + (wF * out[1]).sum()
 
 
+@pytest.mark.parametrize("name", ["bar", "disk"])
 @pytest.mark.parametrize("region", ["grid", "outer", "all"])
-def test_forward_and_xyz_gradient(region):
+def test_forward_and_xyz_gradient(region, name):
     from agama_gpu_amd import CylSplinePotentialGPU
     from agama_gpu_amd.autograd import FittableCylSpline
-    c = _coefs()
-    x = _points(region)
-    print(f"case {region} n={len(x)}", flush=True)
+    c = _coefs(name)
+    x = _points(region, ext=EXT[name])
+    print(f"case {name} {region} n={len(x)}", flush=True)
     m = FittableCylSpline(c, device=DEV)
     ref = CylSplinePotentialGPU(c, device=DEV)
     wphi, wF = _weights(len(x), 5)
@@ -126,13 +134,14 @@ def test_node_and_outer_gradient_directional(region):
         assert gN.abs().max().item() == 0.0
 
 
+@pytest.mark.parametrize("name", ["bar", "disk"])
 @pytest.mark.parametrize("region", ["grid", "outer", "all"])
-def test_table_gradient_directional(region):
+def test_table_gradient_directional(region, name):
     """Through the rebuild: every raw Phi_m(R_i, z_j) entry perturbed."""
     from agama_gpu_amd.autograd import FittableCylSpline
-    c = _coefs()
-    x = _points(region, seed=2)
-    print(f"case {region} n={len(x)}", flush=True)
+    c = _coefs(name)
+    x = _points(region, seed=2, ext=EXT[name])
+    print(f"case {name} {region} n={len(x)}", flush=True)
     m = FittableCylSpline(c, device=DEV)
     xyz = torch.as_tensor(x, device=DEV)
     wphi, wF = _weights(len(x), 13)
