@@ -43,8 +43,45 @@ __device__ __forceinline__ void scatter6(double *g, double s, double a, double b
     }
 }
 
+// Table gradient of one step of the interior evaluation (the hook of
+// interior<ORD_HESS>): per cos/sin term, the gradient of the quintic's six
+// coefficients is s^j alpha + j s^(j-1) beta with
+//   alpha = lam (w_phi Q T + w_theta dQ T + w_phi' X) + w_lnr mu Q T,
+//   beta  = w_lnr lam / dlogr Q T,
+// X the term's d/dphi basis over sin(theta) (its pole limit for m = 1).
+template <bool SLAB>
+struct AdjHook {
+    double *gslab;
+    double *sum;                 // [4]: l > 0 sums of value, d/ds, d/dtheta, d/dphi
+    double s, lam, mu, idl, wphi, wPr, wPt, wPf;
+    int lane;
+    __device__ void side(int off, double Q, double dQ, int m, double T, double dT, double dTp,
+                         double ist, bool pole) const
+    {
+        if (off < 0) return;
+        const double X = pole ? ((m == 1) ? dQ * dTp : 0.0) : Q * dT * ist;
+        const double QT = Q * T;
+        const double alpha = lam * (wphi * QT + wPt * dQ * T + wPf * X) + wPr * mu * QT;
+        const double beta = wPr * lam * idl * QT;
+        scatter6<SLAB>(gslab + off, s, alpha, beta, lane);
+    }
+    __device__ void term(int oc, int os, double Q, double dQ, int m, double cm, double sm,
+                         double ist, bool pole) const
+    {
+        side(oc, Q, dQ, m, cm, -(double)m * sm, -sm, ist, pole);
+        side(os, Q, dQ, m, sm, (double)m * cm, cm, ist, pole);
+    }
+    __device__ void sums(double Sv, double Sr, double St, double Sfn, double Sfp) const
+    {
+        sum[0] = Sv; sum[1] = Sr; sum[2] = St; sum[3] = Sfn;
+        sum[4] = Sfp;
+    }
+};
+
 // One chunk of interval k.  gslab: where this interval's coefficient sums go
 // (LDS slab or memory); the invPhi0 gradient is accumulated per lane in ginv.
+// One pass: the Hessian-vector product's evaluation carries the table
+// gradient's scatter as its per-term hook.
 template <bool SLAB>
 __device__ __forceinline__ void adjoint_interior(const MPK &P, int k, bool valid, double x, double y,
                                                  double z, double wphi, const double wf[3],
@@ -58,16 +95,6 @@ __device__ __forceinline__ void adjoint_interior(const MPK &P, int k, bool valid
     }
     const double wfx = valid ? wf[0] : 0.0, wfy = valid ? wf[1] : 0.0, wfz = valid ? wf[2] : 0.0;
     const double r2 = fma(x, x, fma(y, y, z * z)), r = sqrt(r2), lnr = agx_log(r);
-    // Hessian-vector product for dL/dx
-    {
-        PointOut o{};
-        interior<ORD_HESS, true>(P, slab, x, y, z, r2, r, lnr, k, o, 0.0);
-        if (valid) {
-            gx[0] = wphi * o.g[0] - (o.h[0] * wfx + o.h[3] * wfy + o.h[5] * wfz);
-            gx[1] = wphi * o.g[1] - (o.h[3] * wfx + o.h[1] * wfy + o.h[4] * wfz);
-            gx[2] = wphi * o.g[2] - (o.h[5] * wfx + o.h[4] * wfy + o.h[2] * wfz);
-        }
-    }
     // geometry and seeds: g = Pr u_r + Pt u_t + Pfos u_f
     const double idl = P.inv_dlogr;
     double s = (lnr - fma((double)k, P.dlogr, P.logr_min)) * idl;
@@ -97,50 +124,20 @@ __device__ __forceinline__ void adjoint_interior(const MPK &P, int k, bool valid
         mu = dPdX * c0d * idl;
     }
 
-    // walk the active terms as the forward kernel does
-    double Sv = 0.0, Sr = 0.0, St = 0.0, Sx = 0.0;
-    double cm = 1.0, sm = 0.0, sp = 1.0, sp1 = 0.0;
-    int cur_m = 0;
-    const int S0 = P.n_groups + 1;
-    for (int g = 0; g < P.n_groups; ++g) {
-        const ivec8 gr = cld_rec(P.prog, g);
-        const int m = gr.s0;
-        while (cur_m < m) {
-            sp1 = sp; sp *= st;
-            const double t = fma(cm, cph, -sm * sph);
-            sm = fma(sm, cph, cm * sph); cm = t;
-            ++cur_m;
-        }
-        const double q = rec_lo(gr);
-        double Q1 = q * sp, dQ1 = (m > 0) ? (double)m * q * sp1 * ct : 0.0, Q2 = 0.0, dQ2 = 0.0;
-        const int j0 = gr.s1, j1 = gr.s2;
-        for (int j = j0; j < j1; ++j) {
-            const ivec8 st_rec = cld_rec(P.prog, S0 + j);
-            const int l = st_rec.s0;
-            if (l > m) {
-                const double al = rec_lo(st_rec), be = rec_hi(st_rec);
-                const double Qn = fma(al * ct, Q1, -be * Q2);
-                const double dQn = fma(al, fma(ct, dQ1, -st * Q1), -be * dQ2);
-                Q2 = Q1; Q1 = Qn; dQ2 = dQ1; dQ1 = dQn;
-            }
-            for (int side = 0; side < 2; ++side) {
-                const int off = side == 0 ? st_rec.s1 : st_rec.s2;
-                if (off < 0) continue;
-                const double T = side == 0 ? cm : sm;
-                const double dT = side == 0 ? -(double)m * sm : (double)m * cm;
-                const double dTp = side == 0 ? -sm : cm;        // pole limit, m = 1
-                const double X = pole ? ((m == 1) ? dQ1 * dTp : 0.0) : Q1 * dT * ist;
-                cd_t *a = (cd_t *)(slab + off);
-                const double C = fma(a[5], s5, fma(a[4], s4, fma(a[3], s3, fma(a[2], s2, fma(a[1], s, a[0])))));
-                const double Cd = fma(5.0 * a[5], s4, fma(4.0 * a[4], s3, fma(3.0 * a[3], s2, fma(2.0 * a[2], s, a[1]))));
-                const double QT = Q1 * T;
-                Sv = fma(C, QT, Sv); Sr = fma(Cd, QT, Sr); St = fma(C, dQ1 * T, St); Sx = fma(C, X, Sx);
-                const double alpha = lam * (wphi * QT + wPt * dQ1 * T + wPf * X) + wPr * mu * QT;
-                const double beta = wPr * lam * idl * QT;
-                scatter6<SLAB>(gslab + off, s, alpha, beta, lane);
-            }
+    // evaluation with the Hessian (for dL/dx) and the per-term scatter
+    double sum[5];
+    const AdjHook<SLAB> hook{gslab, sum, s, lam, mu, idl, wphi, wPr, wPt, wPf, lane};
+    {
+        PointOut o{};
+        interior<ORD_HESS, true>(P, slab, x, y, z, r2, r, lnr, k, o, 0.0, hook);
+        if (valid) {
+            gx[0] = wphi * o.g[0] - (o.h[0] * wfx + o.h[3] * wfy + o.h[5] * wfz);
+            gx[1] = wphi * o.g[1] - (o.h[3] * wfx + o.h[1] * wfy + o.h[4] * wfz);
+            gx[2] = wphi * o.g[2] - (o.h[5] * wfx + o.h[4] * wfy + o.h[2] * wfz);
         }
     }
+    const double Sv = sum[0], Sr = sum[1], St = sum[2];
+    const double Sx = pole ? sum[4] : sum[3] * ist;
     // monopole coefficients and invPhi0
     double gc0, gc0d, ginv = 0.0;
     if (P.log_scaling) {
@@ -207,7 +204,8 @@ __device__ __forceinline__ void flush_slab(double *slab, double *g, int cnt, int
 }
 
 // The sorted order, 64 particles of one bin per wavefront and chunk.  Three
-// wavefronts per SIMD (at most 168 VGPRs).
+// wavefronts per SIMD (at most 168 VGPRs: a few spill; measured faster than
+// two wavefronts without spills, 0.757 against 0.79 ms per gradient step).
 template <bool SLAB>
 __global__ __launch_bounds__(kAdjBlock, 3) void mp_adjoint_kernel(
     MPK P, const double *__restrict__ xyz, const double *__restrict__ gphi,

@@ -103,12 +103,22 @@ __device__ __forceinline__ void radial_out(double x, double y, double z, double 
     }
 }
 
+// Per-term hook of interior(): term() sees every step of the Legendre
+// recurrence (cos/sin slab offsets, -1 when absent, P_l^m-scaled value and
+// theta derivative, m, cos/sin(m phi), 1/sin(theta), pole flag) and sums()
+// the l > 0 sums before the log-scaling (value, d/ds, d/dtheta, d/dphi with
+// its pole limit).  The adjoint kernel scatters table gradients through it.
+struct NoHook {
+    __device__ void term(int, int, double, double, int, double, double, double, bool) const {}
+    __device__ void sums(double, double, double, double, double) const {}
+};
+
 // Interior evaluation on radial interval k (slab = poly + k*n_lm*6).
 // UNI: the slab address is wave-uniform (scalar loads of coefficients).
-template <int ORD, bool UNI>
+template <int ORD, bool UNI, class Hook = NoHook>
 __device__ __forceinline__ void interior(const MPK &P, const double *slab, double x, double y,
                                          double z, double r2, double r, double lnr, int k,
-                                         PointOut &o, double inv4piG)
+                                         PointOut &o, double inv4piG, const Hook &hook = Hook{})
 {
     constexpr bool R1 = ORD != ORD_POT;                      // d/d ln r
     constexpr bool D1 = ORD == ORD_GRAD || ORD == ORD_HESS;  // angular first derivatives
@@ -234,6 +244,7 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
                 load6(rec.s2, as);
                 add_term(as, Q, dQ, d2Q, ll1, Vs, Rs, Ts, RRs, TTs, RTs, Ls);
             }
+            hook.term(rec.s1, rec.s2, Q, dQ, m, cm, sm, ist_loop, pole);
         };
         // Qo holds P_{l-2} and becomes P_l; Qn is P_{l-1}
         auto recur = [&](const ivec8 &rec, double &Qo, double &dQo, double Qn, double dQn) {
@@ -293,6 +304,7 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
         gr = gn;
     }
 
+    hook.sums(Sv, Sr, St, Sfn, Sfp);
     // ---- monopole with the log-scaling inverse (after the term loop, so its
     // values are not held in registers across it) ----------------------------
     double a0[6];
