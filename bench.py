@@ -250,7 +250,7 @@ def bench_high_order(dev, n: int = 10_000_000):
             "achieved_tflops": ach, "frac_fp64_peak": ach / PEAK_FP64_TFLOPS, "opcount": info}
 
 
-def bench_grad(dev, reps: int = 5):
+def bench_grad(dev, reps: int = 20, warmup: int = 3):
     """Force fit: Multipole lmax 8 (2 x 81 x 50 live table entries) shifted
     along a 200-knot spline centre, N=1M tracers; one step = loss (mean
     squared force residual) forward + backward to the tables, knots and
@@ -269,7 +269,8 @@ def bench_grad(dev, reps: int = 5):
         loss.backward()
         return loss
 
-    step()
+    for _ in range(warmup):
+        step()
     torch.cuda.synchronize()
     w0 = time.perf_counter()
     for _ in range(reps):
