@@ -59,7 +59,7 @@ __device__ __forceinline__ void cyl_adjoint_outer(const CSK &P, bool valid, doub
         if (m > 0) {
             const double t = fma(cm, cph, -sm * sph);
             sm = fma(sm, cph, cm * sph); cm = t;
-            const double f = -sqrt((2.0 * m + 1.0) / (2.0 * m));
+            const double f = kOuterRec.f[m];
             dpmm = f * fma(ct, pmm, st * dpmm);
             pmm = f * st * pmm;
         }
@@ -69,13 +69,7 @@ __device__ __forceinline__ void cyl_adjoint_outer(const CSK &P, bool valid, doub
         for (int l = 0; l < m; ++l) rl *= q;
         for (int l = m; l <= P.lmax_o; ++l) {
             if (l > m) {
-                double a, b;
-                if (l == m + 1) { a = sqrt(2.0 * m + 3.0); b = 0.0; }
-                else {
-                    const double l2 = (double)l * l, m2 = (double)m * m;
-                    a = sqrt((4.0 * l2 - 1.0) / (l2 - m2));
-                    b = sqrt((2.0 * l + 1.0) * ((l - 1.0) * (l - 1.0) - m2) / ((2.0 * l - 3.0) * (l2 - m2)));
-                }
+                const double a = kOuterRec.a[l][m], b = kOuterRec.b[l][m];
                 const double Qn = fma(a * ct, Q1, -b * Q2);
                 const double dQn = fma(a, fma(ct, dQ1, -st * Q1), -b * dQ2);
                 Q2 = Q1; Q1 = Qn; dQ2 = dQ1; dQ1 = dQn;

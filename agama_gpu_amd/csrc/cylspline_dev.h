@@ -53,6 +53,43 @@ __device__ __forceinline__ void herm1(double t, double h, double fl, double fh, 
     if (D2) d2f = (dh - dl + Q * (t - T)) / h;
 }
 
+// Constants of the outer multipole's normalised Legendre recurrences for
+// l, m <= 8, evaluated at compile time (a square root by Newton's iteration,
+// converged to the last bit): P~_l^m = a ct P~_(l-1)^m - b P~_(l-2)^m, and
+// P~_m^m = f_m sin(theta) P~_(m-1)^(m-1).
+constexpr double const_sqrt(double x)
+{
+    double r = x > 1.0 ? x : 1.0;
+    for (int i = 0; i < 200; ++i) r = 0.5 * (r + x / r);
+    return r;
+}
+
+struct OuterRecurrence {
+    double a[9][9], b[9][9], f[9];   // [l][m], [m]
+};
+
+constexpr OuterRecurrence make_outer_recurrence()
+{
+    OuterRecurrence t{};
+    for (int m = 0; m <= 8; ++m) {
+        t.f[m] = (m == 0) ? 0.0 : -const_sqrt((2.0 * m + 1.0) / (2.0 * m));
+        for (int l = m + 1; l <= 8; ++l) {
+            if (l == m + 1) {
+                t.a[l][m] = const_sqrt(2.0 * m + 3.0);
+                t.b[l][m] = 0.0;
+            } else {
+                const double l2 = (double)l * l, m2 = (double)m * m;
+                t.a[l][m] = const_sqrt((4.0 * l2 - 1.0) / (l2 - m2));
+                t.b[l][m] = const_sqrt((2.0 * l + 1.0) * ((l - 1.0) * (l - 1.0) - m2) /
+                                       ((2.0 * l - 3.0) * (l2 - m2)));
+            }
+        }
+    }
+    return t;
+}
+
+__constant__ const OuterRecurrence kOuterRec = make_outer_recurrence();
+
 // Outer PowerLaw multipole (lmax_o <= 8): spherical derivatives.
 template <bool G, bool H, bool RHO>
 __device__ __forceinline__ void cyl_outer(const CSK &P, double x, double y, double z, double R,
@@ -73,7 +110,7 @@ __device__ __forceinline__ void cyl_outer(const CSK &P, double x, double y, doub
         if (m > 0) {
             const double t = fma(cm, cph, -sm * sph);
             sm = fma(sm, cph, cm * sph); cm = t;
-            const double f = -sqrt((2.0 * m + 1.0) / (2.0 * m));
+            const double f = kOuterRec.f[m];
             dpmm = f * fma(ct, pmm, st * dpmm);   // d/dtheta of f sin(theta) P_(m-1,m-1)
             pmm = f * st * pmm;
         }
@@ -85,13 +122,7 @@ __device__ __forceinline__ void cyl_outer(const CSK &P, double x, double y, doub
         double TTc = 0, TTs = 0, RTc = 0, RTs = 0;
         for (int l = m; l <= P.lmax_o; ++l) {
             if (l > m) {
-                double a, b;
-                if (l == m + 1) { a = sqrt(2.0 * m + 3.0); b = 0.0; }
-                else {
-                    const double l2 = (double)l * l, m2 = (double)m * m;
-                    a = sqrt((4.0 * l2 - 1.0) / (l2 - m2));
-                    b = sqrt((2.0 * l + 1.0) * ((l - 1.0) * (l - 1.0) - m2) / ((2.0 * l - 3.0) * (l2 - m2)));
-                }
+                const double a = kOuterRec.a[l][m], b = kOuterRec.b[l][m];
                 const double Qn = fma(a * ct, Q1, -b * Q2);
                 const double dQn = fma(a, fma(ct, dQ1, -st * Q1), -b * dQ2);
                 Q2 = Q1; Q1 = Qn; dQ2 = dQ1; dQ1 = dQn;
