@@ -30,6 +30,7 @@ class agx_multipole(C.Structure):
         ("n_int", C.c_int32), ("n_lm", C.c_int32), ("n_groups", C.c_int32),
         ("n_steps", C.c_int32), ("lmax", C.c_int32),
         ("log_scaling", C.c_int32), ("inner_log", C.c_int32), ("outer_log", C.c_int32),
+        ("prog_flags", C.c_uint32), ("pad", C.c_int32),
         ("logr_min", C.c_double), ("dlogr", C.c_double), ("inv_dlogr", C.c_double),
         ("invPhi0", C.c_double),
         ("inner_s", C.c_double), ("inner_U", C.c_double), ("inner_W", C.c_double),
@@ -73,6 +74,7 @@ AGX_T_MULTIPOLE, AGX_T_CYLSPLINE = 16, 17
 
 AGX_PHI, AGX_FORCE, AGX_DERIV = 1, 2, 4
 AGX_WORK_SORTED = 1
+AGX_PROG_STRIDE1 = 1
 AGX_OPT_MP_PATH, AGX_OPT_MP_TILE, AGX_OPT_MP_BIN_MIN_N, AGX_OPT_MP_BIN_MIN_TERMS = 1, 2, 3, 4
 AGX_OPT_MP_LDS_GRID = 6
 AGX_OPT_MP_BIN_GRID = 7

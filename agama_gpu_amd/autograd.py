@@ -37,7 +37,7 @@ from .coefs import CylSplineCoefs, MultipoleCoefs
 from .composite import _linear_extrapolating_spline
 from .cylspline_prep import MUL0, MUL1, build_cylspline_data, ptilde
 from .multipole import eval_table
-from .multipole_prep import build_multipole_data, pack_program
+from .multipole_prep import build_multipole_data, pack_program, program_flags
 
 F64 = torch.float64
 
@@ -197,6 +197,7 @@ class FittableMultipole(torch.nn.Module):
         prog = d["program"]
         self._program = torch.as_tensor(pack_program(prog).view(np.uint8), device=dev)
         self._n_groups, self._n_steps = len(prog.grp_m), len(prog.step_l)
+        self._prog_flags = program_flags(prog)
         self._cols32 = torch.as_tensor(np.asarray(d["columns"], np.int32), device=dev)
         self._modes = ({"log": 1, "s2": 2}.get(d["inner_mode"], 0), int(bool(d["inner_reset"])),
                        1 if d["outer_mode"] == "log" else 0)
@@ -278,6 +279,7 @@ class FittableMultipole(torch.nn.Module):
         s.program = self._program.data_ptr()
         s.n_int, s.n_lm = d["n_intervals"], d["n_lm"]
         s.n_groups, s.n_steps, s.lmax = self._n_groups, self._n_steps, d["lmax"]
+        s.prog_flags = self._prog_flags
         s.log_scaling, s.inner_log, s.outer_log = d["log_scaling"], d["inner_log"], d["outer_log"]
         s.logr_min, s.dlogr, s.inv_dlogr = d["logr_min"], d["dlogr"], d["inv_dlogr"]
         s.inner_s, s.outer_s = d["inner_s"], d["outer_s"]

@@ -49,6 +49,17 @@ __device__ __forceinline__ double rec_hi(const ivec8 &v)
     t.y = v.s7;
     return __builtin_bit_cast(double, t);
 }
+// FP64 in dwords 2-3 of a record.
+__device__ __forceinline__ double rec_mid(const ivec8 &v)
+{
+    ivec2 t;
+    t.x = v.s2;
+    t.y = v.s3;
+    return __builtin_bit_cast(double, t);
+}
+// The two int16 halves of dword 1 (sign-extended): low and high.
+__device__ __forceinline__ int rec_off_lo(const ivec8 &v) { return (int)(short)(v.s1 & 0xFFFF); }
+__device__ __forceinline__ int rec_off_hi(const ivec8 &v) { return v.s1 >> 16; }
 
 // 1/x from the hardware reciprocal estimate refined by two Newton steps
 // (within 2 ulp of the rounded quotient for normal x; the IEEE division

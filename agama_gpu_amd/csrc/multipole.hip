@@ -52,7 +52,8 @@ __global__ __launch_bounds__(kBinThreads) void mp_binned_kernel(
 // per workgroup: the per-term gathers of the lane kernel become LDS reads.
 // Used for tables of up to 160 KB (the LDS of one CU); the workgroup size is
 // chosen so that the workgroups that fit in LDS hold 16 wavefronts per CU.
-template <int ORD, int BLOCK>
+// Without STRIDE2 the program must not recur by 2 (AGX_PROG_STRIDE1).
+template <int ORD, int BLOCK, bool STRIDE2>
 __global__ __launch_bounds__(BLOCK) void mp_lds_kernel(MPK P, const double *__restrict__ xyz,
                                                         int64_t n, double *__restrict__ phi,
                                                         double *__restrict__ force,
@@ -80,8 +81,8 @@ __global__ __launch_bounds__(BLOCK) void mp_lds_kernel(MPK P, const double *__re
         PointOut o{};
         Where w = classify(P, x, y, z);
         if (w.kind == 2)
-            interior<ORD, false>(P, lds_tab + w.k * slab_len, x, y, z, w.r2, w.r, w.lnr, w.k, o,
-                                 inv4piG);
+            interior<ORD, false, NoHook, STRIDE2>(P, lds_tab + w.k * slab_len, x, y, z, w.r2,
+                                                  w.r, w.lnr, w.k, o, inv4piG);
         else
             extrap<ORD>(P, w, x, y, z, o, inv4piG);
         store<ORD>(o, i, phi, force, deriv);
@@ -125,6 +126,7 @@ static int check_mp(const agx_multipole *mp, const double *xyz, int64_t n)
     if (!mp->poly_km || !mp->program) return AGX_E_NULL;
     if (mp->n_int < 1 || mp->n_lm < 1 || mp->n_groups < 0 || mp->n_steps < 0) return AGX_E_SIZE;
     if (mp->lmax > 32) return AGX_E_LMAX;
+    if (mp->prog_flags & ~AGX_PROG_STRIDE1) return AGX_E_WHAT;
     return AGX_OK;
 }
 
@@ -183,16 +185,36 @@ static unsigned lds_grid(int64_t n, int block, int64_t per_cu)
 }
 
 // Dynamic LDS beyond 64 KB has to be allowed per kernel.
-template <int ORD, int BLOCK>
+template <int ORD, int BLOCK, bool STRIDE2>
 static void lds_attr(int64_t bytes)
 {
     static int64_t set = 0;
     if (bytes > set) {
-        if (hipFuncSetAttribute((const void *)mp_lds_kernel<ORD, BLOCK>,
+        if (hipFuncSetAttribute((const void *)mp_lds_kernel<ORD, BLOCK, STRIDE2>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsPerCu) ==
             hipSuccess)
             set = kLdsPerCu;
     }
+}
+
+template <int ORD, int BLOCK, bool STRIDE2>
+static void launch_lds(const MPK &P, unsigned grid, int64_t tb, const double *xyz, int64_t n,
+                       double *phi, double *force, double *deriv, double inv4piG, hipStream_t s)
+{
+    if (BLOCK > 256) lds_attr<ORD, BLOCK, STRIDE2>(tb);
+    hipLaunchKernelGGL((mp_lds_kernel<ORD, BLOCK, STRIDE2>), dim3(grid), dim3(BLOCK), (size_t)tb,
+                       s, P, xyz, n, phi, force, deriv, inv4piG);
+}
+
+template <int ORD, int BLOCK>
+static void launch_lds(const agx_multipole *mp, const MPK &P, unsigned grid, int64_t tb,
+                       const double *xyz, int64_t n, double *phi, double *force, double *deriv,
+                       double inv4piG, hipStream_t s)
+{
+    if (mp->prog_flags & AGX_PROG_STRIDE1)
+        launch_lds<ORD, BLOCK, false>(P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s);
+    else
+        launch_lds<ORD, BLOCK, true>(P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s);
 }
 
 template <int ORD>
@@ -220,21 +242,15 @@ static void launch_mp(const agx_multipole *mp, const MPK &P, const double *xyz, 
         // oversubscribed and the dispatcher hands freed slots to new
         // workgroups; each one re-stages the table from L2.
         const int64_t per_cu = fit * g_mp_lds_grid;
-        if (fit >= 4) {
-            const unsigned grid = grid_for(n, (int)per_cu);
-            hipLaunchKernelGGL((mp_lds_kernel<ORD, 256>), dim3(grid), dim3(256), (size_t)tb, s, P,
-                               xyz, n, phi, force, deriv, inv4piG);
-        } else if (fit >= 2) {
-            const unsigned grid = lds_grid(n, 512, per_cu);
-            lds_attr<ORD, 512>(tb);
-            hipLaunchKernelGGL((mp_lds_kernel<ORD, 512>), dim3(grid), dim3(512), (size_t)tb, s, P,
-                               xyz, n, phi, force, deriv, inv4piG);
-        } else {
-            const unsigned grid = lds_grid(n, 1024, per_cu);
-            lds_attr<ORD, 1024>(tb);
-            hipLaunchKernelGGL((mp_lds_kernel<ORD, 1024>), dim3(grid), dim3(1024), (size_t)tb, s, P,
-                               xyz, n, phi, force, deriv, inv4piG);
-        }
+        if (fit >= 4)
+            launch_lds<ORD, 256>(mp, P, grid_for(n, (int)per_cu), tb, xyz, n, phi, force, deriv,
+                                 inv4piG, s);
+        else if (fit >= 2)
+            launch_lds<ORD, 512>(mp, P, lds_grid(n, 512, per_cu), tb, xyz, n, phi, force, deriv,
+                                 inv4piG, s);
+        else
+            launch_lds<ORD, 1024>(mp, P, lds_grid(n, 1024, per_cu), tb, xyz, n, phi, force, deriv,
+                                  inv4piG, s);
     } else if (path == 2) {
         BinLaunch bl = bin_launch(n);
         hipLaunchKernelGGL(mp_binned_kernel<ORD>, dim3(bl.grid), dim3(kBinThreads), 0, s, P, xyz,

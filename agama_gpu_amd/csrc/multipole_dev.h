@@ -115,7 +115,7 @@ struct NoHook {
 
 // Interior evaluation on radial interval k (slab = poly + k*n_lm*6).
 // UNI: the slab address is wave-uniform (scalar loads of coefficients).
-template <int ORD, bool UNI, class Hook = NoHook>
+template <int ORD, bool UNI, class Hook = NoHook, bool STRIDE2 = true>
 __device__ __forceinline__ void interior(const MPK &P, const double *slab, double x, double y,
                                          double z, double r2, double r, double lnr, int k,
                                          PointOut &o, double inv4piG, const Hook &hook = Hook{})
@@ -225,50 +225,67 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
         // the cos coefficients were loaded into ca beforehand
         auto terms = [&](const ivec8 &rec, double Q, double dQ, const double ca[6]) {
             const int l = rec.s0;
+            const int oc = rec_off_lo(rec), os = rec_off_hi(rec);
             double d2Q = 0.0;
             if (ANG2 && !pole)
                 d2Q = -ct * ist_loop * dQ -
                       ((double)(l * (l + 1)) - (double)(m * m) * ist_loop * ist_loop) * Q;
             const double ll1 = (double)(l * (l + 1));
-            if (rec.s1 >= 0) {
+            if (oc >= 0) {
                 if (UNI) {
                     add_term(ca, Q, dQ, d2Q, ll1, Vc, Rc, Tc, RRc, TTc, RTc, Lc);
                 } else {
                     double ac[6];
-                    load6(rec.s1, ac);
+                    load6(oc, ac);
                     add_term(ac, Q, dQ, d2Q, ll1, Vc, Rc, Tc, RRc, TTc, RTc, Lc);
                 }
             }
-            if (rec.s2 >= 0) {
+            if (os >= 0) {
                 double as[6];
-                load6(rec.s2, as);
+                load6(os, as);
                 add_term(as, Q, dQ, d2Q, ll1, Vs, Rs, Ts, RRs, TTs, RTs, Ls);
             }
-            hook.term(rec.s1, rec.s2, Q, dQ, m, cm, sm, ist_loop, pole);
+            hook.term(oc, os, Q, dQ, m, cm, sm, ist_loop, pole);
         };
-        // Qo holds P_{l-2} and becomes P_l; Qn is P_{l-1}
+        // One step of stride s (1, or 2 when the group's terms all have l - m
+        // even): Qo holds P_{l-2s} and becomes P_l, Qn is P_{l-s}, and with
+        // c = cos (s = 1) or cos^2 (s = 2) and t = alpha (c + beta)
+        //   P_l = t P_{l-s} + gamma P_{l-2s},
+        //   dP_l/dtheta = t dP_{l-s} + alpha (dc/dtheta) P_{l-s} + gamma dP_{l-2s};
+        // beta is 0 where s = 1.
+        // c and dc/dtheta from h = s - 1 by arithmetic rather than a select,
+        // so that no cos^2 or cos sin stays live across the group loop;
+        // without STRIDE2 (no group recurs by 2) c = cos, dc = -sin
+        double cc = ct, dcc = -st;
+        if (STRIDE2) {
+            const double h = (double)(gr.s3 - 1);
+            const double u = fma(ct, h, 1.0 - h);              // cos or 1
+            cc = ct * u;
+            dcc = -st * fma(h, ct, u);
+        }
         auto recur = [&](const ivec8 &rec, double &Qo, double &dQo, double Qn, double dQn) {
-            const double al = rec_lo(rec), be = rec_hi(rec);
-            if (D1) dQo = fma(al, fma(ct, dQn, -st * Qn), -be * dQo);
-            Qo = fma(al * ct, Qn, -be * Qo);
+            const double al = rec_lo(rec), be = rec_hi(rec), ga = rec_mid(rec);
+            const double t = STRIDE2 ? al * (cc + be) : al * cc;
+            if (D1) dQo = fma(t, dQn, fma(al * dcc, Qn, ga * dQo));
+            Qo = fma(t, Qn, ga * Qo);
         };
         // Steps j0 .. j1-1 (the first has l = m).  Records past the group are
         // read (the next group's, or the zero record) but not used.
         const int j0 = gr.s1, j1 = gr.s2;
         double cA[6], cB[6];
         const ivec8 r0 = cld_rec(P.prog, S0 + j0);
-        if (UNI && r0.s1 >= 0) load6(r0.s1, cA);
+        if (UNI && rec_off_lo(r0) >= 0) load6(rec_off_lo(r0), cA);
         ivec8 r1 = cld_rec(P.prog, S0 + j0 + 1);
-        if (UNI && j0 + 1 < j1 && r1.s1 >= 0) load6(r1.s1, cB);
+        if (UNI && j0 + 1 < j1 && rec_off_lo(r1) >= 0) load6(rec_off_lo(r1), cB);
         terms(r0, QA, dQA, cA);
         int j = j0 + 1;
         for (; j + 1 < j1; j += 2) {
             const ivec8 r2 = cld_rec(P.prog, S0 + j + 1);
-            if (UNI && r2.s1 >= 0) load6(r2.s1, cA);
+            if (UNI && rec_off_lo(r2) >= 0) load6(rec_off_lo(r2), cA);
             recur(r1, QB, dQB, QA, dQA);
             terms(r1, QB, dQB, cB);
             const ivec8 r3 = cld_rec(P.prog, S0 + j + 2);
-            if (UNI && j + 2 < j1 && r3.s1 >= 0) load6(r3.s1, cB);
+            if (UNI && j + 2 < j1 && rec_off_lo(r3) >= 0) load6(rec_off_lo(r3), cB);
             recur(r2, QA, dQA, QB, dQB);
             terms(r2, QA, dQA, cA);
             r1 = r3;

@@ -20,7 +20,7 @@ import torch
 from . import _lib
 from .base import PotentialBase, alloc_outputs, ptr
 from .coefs import MultipoleCoefs, read_coefs
-from .multipole_prep import INV_4PIG, build_multipole_data, pack_program
+from .multipole_prep import INV_4PIG, build_multipole_data, pack_program, program_flags
 
 
 # Tables above this size do not fit the LDS-staged path; from this many points
@@ -73,6 +73,7 @@ class MultipolePotentialGPU(PotentialBase):
         s.n_lm = d["n_lm"]
         s.n_groups = len(prog.grp_m)
         s.n_steps = len(prog.step_l)
+        s.prog_flags = program_flags(prog)
         s.lmax = d["lmax"]
         s.log_scaling = d["log_scaling"]
         s.inner_log = d["inner_log"]

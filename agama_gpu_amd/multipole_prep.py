@@ -153,7 +153,14 @@ def _sort_key_order(lm_l: np.ndarray, lm_m: np.ndarray) -> np.ndarray:
 
 @dataclass
 class LegendreProgram:
-    """Per-potential recurrence program consumed by the Multipole kernels."""
+    """Per-potential recurrence program consumed by the Multipole kernels.
+
+    Every step is P_l = alpha (c + beta) P_{l-s} + gamma P_{l-2s}.  A group
+    whose active terms all have l - |m| even steps l by s = 2 with c = cos^2
+    (the same-parity recurrence); the others step by s = 1 with c = cos, beta
+    = 0 and gamma = minus the one-step recurrence's beta.  (alpha times a sum
+    rather than alpha c + beta: one coefficient per instruction, the most a
+    gfx950 FP64 instruction takes from scalar registers.)"""
     grp_m: np.ndarray        # int32 (G,)
     grp_step0: np.ndarray    # int32 (G+1,)  steps of group g: [step0[g], step0[g+1])
     qmm: np.ndarray          # float64 (G,)  mul_m * normalised P_m^m / sin^m
@@ -162,6 +169,8 @@ class LegendreProgram:
     step_cos: np.ndarray     # int32 (S,)  table column of the cos term, -1 if none
     step_sin: np.ndarray     # int32 (S,)
     step_l: np.ndarray       # int32 (S,)
+    grp_stride: np.ndarray   # int32 (G,): 1 or 2
+    step_gamma: np.ndarray   # float64 (S,)
 
 
 def _qmm(m: int) -> float:
@@ -174,34 +183,61 @@ def _qmm(m: int) -> float:
     return mul * v
 
 
+def _recurrence_ab(l: int, am: int):
+    """alpha_l, beta_l of the normalised one-step recurrence
+    P_l = alpha_l cos P_{l-1} - beta_l P_{l-2} (l > |m|)."""
+    if l == am + 1:
+        return math.sqrt(2.0 * am + 3.0), 0.0
+    l2, m2 = float(l * l), float(am * am)
+    return (math.sqrt((4.0 * l2 - 1.0) / (l2 - m2)),
+            math.sqrt((2.0 * l + 1.0) * ((l - 1.0) ** 2 - m2) / ((2.0 * l - 3.0) * (l2 - m2))))
+
+
+def _recurrence2(l: int, am: int):
+    """(alpha, beta, gamma) of the two-step recurrence
+    P_l = (alpha cos^2 + beta) P_{l-2} + gamma P_{l-4}, l >= |m| + 2: the
+    one-step recurrence applied twice, with cos P_{l-3} eliminated through
+    P_{l-2} = alpha_{l-2} cos P_{l-3} - beta_{l-2} P_{l-4}."""
+    a_l, b_l = _recurrence_ab(l, am)
+    a_1, b_1 = _recurrence_ab(l - 1, am)
+    if l == am + 2:
+        return a_l * a_1, -b_l, 0.0
+    a_2, b_2 = _recurrence_ab(l - 2, am)
+    return a_l * a_1, -(a_l * b_1 / a_2 + b_l), -a_l * b_1 * b_2 / a_2
+
+
 def build_legendre_program(lm_l: np.ndarray, lm_m: np.ndarray) -> LegendreProgram:
     """Group the sorted active harmonics (column 0 = (0,0) excluded: the
-    kernels treat the monopole separately) into |m| runs."""
+    kernels treat the monopole separately) into |m| runs; a run whose terms
+    all have l - |m| even steps by 2 (the skipped odd steps carry no term)."""
     cols: Dict[int, Dict[int, List[int]]] = {}
     for c in range(1, len(lm_l)):
         l, m = int(lm_l[c]), int(lm_m[c])
         am = abs(m)
         slot = cols.setdefault(am, {}).setdefault(l, [-1, -1])
         slot[0 if m >= 0 else 1] = c
-    grp_m, grp_step0, qmm = [], [0], []
-    alpha, beta, scos, ssin, sl = [], [], [], [], []
+    grp_m, grp_step0, qmm, stride = [], [0], [], []
+    alpha, beta, gamma, scos, ssin, sl = [], [], [], [], [], []
     for am in sorted(cols):
         ls = cols[am]
         l_last = max(ls)
+        two = all((l - am) % 2 == 0 for l in ls)
         grp_m.append(am)
         qmm.append(_qmm(am))
-        for l in range(am, l_last + 1):
+        stride.append(2 if two else 1)
+        for l in range(am, l_last + 1, 2 if two else 1):
             if l == am:
-                a, b = 0.0, 0.0
-            elif l == am + 1:
-                a, b = math.sqrt(2.0 * am + 3.0), 0.0
+                a, b, g = 0.0, 0.0, 0.0
+            elif two:
+                a, b, g = _recurrence2(l, am)
+                b /= a
             else:
-                l2, m2 = float(l * l), float(am * am)
-                a = math.sqrt((4.0 * l2 - 1.0) / (l2 - m2))
-                b = math.sqrt((2.0 * l + 1.0) * ((l - 1.0) ** 2 - m2) / ((2.0 * l - 3.0) * (l2 - m2)))
+                a, b1 = _recurrence_ab(l, am)
+                b, g = 0.0, -b1
             c_cos, c_sin = ls.get(l, [-1, -1])
             alpha.append(a)
             beta.append(b)
+            gamma.append(g)
             scos.append(c_cos)
             ssin.append(c_sin)
             sl.append(l)
@@ -210,11 +246,38 @@ def build_legendre_program(lm_l: np.ndarray, lm_m: np.ndarray) -> LegendreProgra
         np.asarray(grp_m, np.int32), np.asarray(grp_step0, np.int32),
         np.asarray(qmm, np.float64), np.asarray(alpha, np.float64),
         np.asarray(beta, np.float64), np.asarray(scos, np.int32),
-        np.asarray(ssin, np.int32), np.asarray(sl, np.int32))
+        np.asarray(ssin, np.int32), np.asarray(sl, np.int32),
+        np.asarray(stride, np.int32), np.asarray(gamma, np.float64))
 
 
-PROGRAM_RECORD = np.dtype([("i0", "<i4"), ("i1", "<i4"), ("i2", "<i4"), ("i3", "<i4"),
-                           ("d0", "<f8"), ("d1", "<f8")])
+def program_flags(prog: LegendreProgram) -> int:
+    """agx_multipole.prog_flags for this program: AGX_PROG_STRIDE1 (1) when
+    every group that recurs (more than one step) has stride 1."""
+    steps = np.diff(prog.grp_step0)
+    return int(not np.any((prog.grp_stride == 2) & (steps > 1)))
+
+
+# 32 bytes: four int32 (group records: m, step_begin, step_end, stride; step
+# records: l, the packed offsets and, over i2-i3, the double gamma), then two
+# doubles (group: qmm, 0; step: alpha, beta).  The step record of l = m is all
+# coefficients zero.
+PROGRAM_RECORD = np.dtype({"names": ["i0", "i1", "i2", "i3", "g", "d0", "d1"],
+                           "formats": ["<i4", "<i4", "<i4", "<i4", "<f8", "<f8", "<f8"],
+                           "offsets": [0, 4, 8, 12, 8, 16, 24], "itemsize": 32})
+
+
+def pack_offsets(cos_off: np.ndarray, sin_off: np.ndarray) -> np.ndarray:
+    """int32 holding the cos offset (int16, low half) and the sin offset (int16,
+    high half); -1 marks an absent term."""
+    lo = cos_off.astype(np.int64) & 0xFFFF
+    hi = (sin_off.astype(np.int64) & 0xFFFF) << 16
+    return (lo | hi).astype(np.uint32).view(np.int32)
+
+
+def unpack_offsets(packed: np.ndarray):
+    p = np.asarray(packed, np.int32)
+    return (p.astype(np.uint32) & 0xFFFF).astype(np.uint16).view(np.int16).astype(np.int32), \
+        (p >> 16).astype(np.int32)
 
 
 def pack_program(prog: LegendreProgram) -> np.ndarray:
@@ -222,16 +285,20 @@ def pack_program(prog: LegendreProgram) -> np.ndarray:
     include/agama_gpu.h): group records, a zero record, step records (with
     column offsets 6*c), a zero record."""
     G, S = len(prog.grp_m), len(prog.step_l)
+    if S and 6 * (max(int(prog.step_cos.max()), int(prog.step_sin.max())) + 1) > 32767:
+        raise ValueError("too many terms for 16-bit slab offsets")
     rec = np.zeros(G + S + 2, PROGRAM_RECORD)
     g = rec[:G]
     g["i0"] = prog.grp_m
     g["i1"] = prog.grp_step0[:-1]
     g["i2"] = prog.grp_step0[1:]
+    g["i3"] = prog.grp_stride
     g["d0"] = prog.qmm
     st = rec[G + 1:G + 1 + S]
     st["i0"] = prog.step_l
-    st["i1"] = np.where(prog.step_cos >= 0, 6 * prog.step_cos, -1)
-    st["i2"] = np.where(prog.step_sin >= 0, 6 * prog.step_sin, -1)
+    st["i1"] = pack_offsets(np.where(prog.step_cos >= 0, 6 * prog.step_cos, -1),
+                            np.where(prog.step_sin >= 0, 6 * prog.step_sin, -1))
+    st["g"] = prog.step_gamma
     st["d0"] = prog.step_alpha
     st["d1"] = prog.step_beta
     return rec

@@ -46,12 +46,15 @@ def opcount_multipole(desc: dict, kinds: np.ndarray, order: int = 1) -> float:
     terms including (0,0), G active |m| groups and S = sum over groups of
     (l_last - |m|) recurrence steps; pot only: 60 + 16 n + 6 S; evalDeriv:
     350 + 90 n + 12 S + 5 G.  Extrapolated point: 40 (pot+force), 60
-    (evalDeriv), 20 (pot).
+    (evalDeriv), 20 (pot).  The count is that of the one-step recurrence
+    for every table; the kernels step by 2 in groups whose terms all have
+    l - |m| even, and run S_executed steps (reported beside S).
     """
     prog = desc["program"]
     n = desc["n_lm"]
     G = len(prog.grp_m)
     S = int(sum(int(prog.step_l[prog.grp_step0[g + 1] - 1]) - int(prog.grp_m[g]) for g in range(G)))
+    S_exec = int(np.sum(np.diff(prog.grp_step0) - 1))
     if order == 0:
         f_in, f_ex = 60 + 16 * n + 6 * S, 20
     elif order == 1:
@@ -59,8 +62,9 @@ def opcount_multipole(desc: dict, kinds: np.ndarray, order: int = 1) -> float:
     else:
         f_in, f_ex = 350 + 90 * n + 12 * S + 5 * G, 60
     n_in = int(np.count_nonzero(kinds == 2))
-    return float(f_in * n_in + f_ex * (len(kinds) - n_in)), dict(n=n, G=G, S=S, f_interior=f_in,
-                                                                   f_extrap=f_ex, n_interior=n_in)
+    return float(f_in * n_in + f_ex * (len(kinds) - n_in)), dict(n=n, G=G, S=S, S_executed=S_exec,
+                                                                   f_interior=f_in, f_extrap=f_ex,
+                                                                   n_interior=n_in)
 
 
 def classify(desc: dict, x: np.ndarray) -> np.ndarray:

@@ -54,23 +54,38 @@ enum {
  *   program : the Legendre recurrence schedule, 32-byte records (device,
  *             16-byte aligned), n_groups + n_steps + 2 of them:
  *       [0, n_groups)   one per active |m| group, in increasing |m|:
- *                       int32 m, step_begin, step_end, 0;
+ *                       int32 m, step_begin, step_end, stride s (1 or 2);
  *                       double qmm (mul_m x normalised P_mm / sin^m theta), 0
  *       n_groups        all zero
  *       [n_groups + 1, n_groups + 1 + n_steps)   one per recurrence step
- *                       l = m .. l_last of each group, groups in order:
- *                       int32 l, cos_off, sin_off, 0; double alpha_l, beta_l
- *                       (P_l = alpha_l cos(theta) P_{l-1} - beta_l P_{l-2};
+ *                       l = m, m + s, .. l_last of each group, in order:
+ *                       int32 l, int32 offsets (int16 cos_off in the low
+ *                       half, int16 sin_off in the high half), double gamma;
+ *                       double alpha, beta
+ *                       (P_l = alpha (c + beta) P_{l-s} + gamma P_{l-2s},
+ *                       c = cos(theta) for s = 1, cos^2(theta) for s = 2;
+ *                       beta = 0 where s = 1; s = 2 only where every
+ *                       active term of the group has l - m even; the l = m
+ *                       record's coefficients are 0;
  *                       cos_off / sin_off = 6 x column of the (l, +m) /
  *                       (l, -m) term in a poly_km slab, -1 if not active)
  *       last            all zero
  *             step_begin/step_end index the step records from 0.
+ *   prog_flags : 0, or AGX_PROG_STRIDE1 when every group of `program` with
+ *             more than one step has stride 1; the LDS-staged kernels then
+ *             run a variant without the stride-2 arithmetic.  0 is always
+ *             correct; setting the flag for a program that does step by 2
+ *             gives wrong results.
  * ---------------------------------------------------------------------- */
+#define AGX_PROG_STRIDE1 1u
+
 typedef struct agx_multipole {
     const double *poly_km;
     const void *program;
     int32_t n_int, n_lm, n_groups, n_steps, lmax;
     int32_t log_scaling, inner_log, outer_log;
+    uint32_t prog_flags;
+    int32_t pad;
     double logr_min, dlogr, inv_dlogr, invPhi0;
     double inner_s, inner_U, inner_W;
     double outer_s, outer_U, outer_W;

@@ -55,6 +55,23 @@ def test_argument_validation_without_gpu():
     assert L.agx_multipole_eval(C.byref(d), 8, 10, 8, None, None, 0, None) == -5
     assert L.agx_multipole_eval(C.byref(d), 8, 10, None, None, None, 1, None) == -1
     assert L.agx_multipole_eval(C.byref(d), 8, 0, 8, None, None, 1, None) == 0
+    d.prog_flags = _lib.AGX_PROG_STRIDE1
+    assert L.agx_multipole_eval(C.byref(d), 8, 0, 8, None, None, 1, None) == 0
+    d.prog_flags = 2                    # no such program flag
+    assert L.agx_multipole_eval(C.byref(d), 8, 10, 8, None, None, 1, None) == -5
+
+
+def test_descriptor_layout_matches_header():
+    """The ctypes mirror of agx_multipole has the header's field order and
+    C layout (no implicit padding before the doubles)."""
+    from agama_gpu_amd import _lib
+    text = (ROOT / "include" / "agama_gpu.h").read_text()
+    body = re.search(r"typedef struct agx_multipole \{(.*?)\} agx_multipole;", text, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_0-9]+)\s*[,;]", body)
+    assert names == [f[0] for f in _lib.agx_multipole._fields_]
+    assert _lib.agx_multipole.logr_min.offset == 56
+    assert C.sizeof(_lib.agx_multipole) == 56 + 10 * 8 + 8
 
 
 def test_adjoint_workspace_validation_without_gpu():

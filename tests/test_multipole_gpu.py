@@ -113,3 +113,31 @@ def test_empty_and_ragged_sizes(cases, mp_path):
         po, Fo = orc.pot_force(x)
         assert rel_phi(p.cpu().numpy(), po) < TOL_PHI
         assert rel_glob(F.cpu().numpy(), Fo) < TOL_F
+
+
+@pytest.mark.parametrize("name", ["fire_dense", "lmc", "halo_lmax8"])
+def test_program_flag_variants_agree(cases, name):
+    """The LDS-staged kernel without the stride-2 arithmetic (taken when the
+    descriptor carries AGX_PROG_STRIDE1) and the generic one give bit-equal
+    results on programs that recur by 1: for such groups the generic step's
+    c = cos * fma(cos, 0, 1) and dc = -sin * fma(0, cos, 1) are exact.  The
+    flag is set exactly when no group recurs by 2 (halo_lmax8: every |m| group
+    steps by 2, so the flag is clear)."""
+    from agama_gpu_amd import MultipolePotentialGPU, _lib
+    from agama_gpu_amd.multipole_prep import program_flags
+    gpu = MultipolePotentialGPU(cases[name])
+    prog = gpu._data["program"]
+    want = 0 if name == "halo_lmax8" else _lib.AGX_PROG_STRIDE1
+    assert program_flags(prog) == want == gpu._desc.prog_flags
+    x = torch.as_tensor(points(100_000, 4), device="cuda")
+    _lib.set_option(_lib.AGX_OPT_MP_PATH, 3)
+    try:
+        outs = []
+        for flags in (gpu._desc.prog_flags, 0):
+            gpu._desc.prog_flags = flags
+            outs.append(gpu.evalDeriv(x))
+    finally:
+        gpu._desc.prog_flags = want
+        _lib.set_option(_lib.AGX_OPT_MP_PATH, 0)
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)

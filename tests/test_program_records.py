@@ -21,9 +21,10 @@ def _program(lmax, sym="none"):
     return l[o], m[o], build_legendre_program(l[o], m[o])
 
 
-@pytest.mark.parametrize("lmax,sym", [(8, "triaxial"), (8, "none"), (32, "none")])
+@pytest.mark.parametrize("lmax,sym", [(8, "triaxial"), (8, "none"), (32, "none"),
+                                      (32, "triaxial")])
 def test_records_encode_program(lmax, sym):
-    from agama_gpu_amd.multipole_prep import PROGRAM_RECORD, pack_program
+    from agama_gpu_amd.multipole_prep import PROGRAM_RECORD, pack_program, unpack_offsets
     l, m, prog = _program(lmax, sym)
     rec = pack_program(prog)
     assert rec.dtype == PROGRAM_RECORD and rec.dtype.itemsize == 32
@@ -32,12 +33,16 @@ def test_records_encode_program(lmax, sym):
     assert not any(rec[G].tolist()) and not any(rec[-1].tolist())
     seen = set()
     for g in range(G):
-        mm, j0, j1 = rec[g]["i0"], rec[g]["i1"], rec[g]["i2"]
+        mm, j0, j1, stride = rec[g]["i0"], rec[g]["i1"], rec[g]["i2"], rec[g]["i3"]
+        # stride 2 exactly when every term of the group has l - m even
+        ls = [int(a) for a, b in zip(l[1:], m[1:]) if abs(int(b)) == mm]
+        assert stride == (2 if all((a - mm) % 2 == 0 for a in ls) else 1)
         assert rec[g]["d0"] == prog.qmm[g]
         for j in range(j0, j1):
             s = rec[G + 1 + j]
-            assert s["i0"] == mm + (j - j0)
-            for off, sign in ((s["i1"], 1), (s["i2"], -1)):
+            assert s["i0"] == mm + stride * (j - j0)
+            oc, os_ = unpack_offsets(np.array([s["i1"]]))
+            for off, sign in ((int(oc[0]), 1), (int(os_[0]), -1)):
                 if off < 0:
                     continue
                 assert off % 6 == 0
@@ -47,25 +52,44 @@ def test_records_encode_program(lmax, sym):
     assert seen == set(range(1, len(l)))          # every non-monopole term once
 
 
-def test_records_reproduce_legendre():
-    """Run the recurrence from the records at a few cos(theta) and compare
-    mul_m x normalised P_l^m (Condon-Shortley) from scipy."""
+@pytest.mark.parametrize("lmax,sym", [(12, "none"), (12, "triaxial"), (32, "triaxial")])
+def test_records_reproduce_legendre(lmax, sym):
+    """Run the recurrence from the records at a few theta and compare
+    mul_m x normalised P_l^m (Condon-Shortley) from scipy, and its theta
+    derivative with a central difference of the same.  Every step is
+    P_l = alpha (c + beta) P_{l-s} + gamma P_{l-2s} with c = cos theta for
+    stride s = 1 (where beta = 0) and cos^2 theta for the two-step
+    (same-parity) groups, differentiated as dP_l = alpha (c + beta) dP_{l-s}
+    + alpha c' P_{l-s} + gamma dP_{l-2s}."""
     from agama_gpu_amd.multipole_prep import pack_program
-    _, _, prog = _program(12, "none")
+    _, _, prog = _program(lmax, sym)
     rec = pack_program(prog)
     G = len(prog.grp_m)
-    for ct in (-0.93, -0.2, 0.0, 0.41, 0.99):
-        st = math.sqrt(1 - ct * ct)
+
+    def want(ll, mm, th):
+        norm = math.sqrt((2 * ll + 1) / (4 * math.pi) * math.factorial(ll - mm) /
+                         math.factorial(ll + mm))
+        mul = 2 * math.sqrt(math.pi) if mm == 0 else 2 * math.sqrt(2 * math.pi)
+        return mul * norm * lpmv(mm, ll, math.cos(th))
+
+    for th in (2.76, 1.77, 0.5 * math.pi, 1.15, 0.14):
+        ct, st = math.cos(th), math.sin(th)
         for g in range(G):
             mm, j0, j1, q = rec[g]["i0"], rec[g]["i1"], rec[g]["i2"], rec[g]["d0"]
+            two = rec[g]["i3"] == 2
+            c, dc = (ct * ct, -2 * ct * st) if two else (ct, -st)
             Q1, Q2 = q * st ** mm, 0.0
+            dQ1, dQ2 = q * mm * st ** (mm - 1) * ct if mm else 0.0, 0.0
             for j in range(j0, j1):
                 s = rec[G + 1 + j]
                 ll = s["i0"]
                 if ll > mm:
-                    Q1, Q2 = s["d0"] * ct * Q1 - s["d1"] * Q2, Q1
-                norm = math.sqrt((2 * ll + 1) / (4 * math.pi) * math.factorial(ll - mm) /
-                                 math.factorial(ll + mm))
-                mul = 2 * math.sqrt(math.pi) if mm == 0 else 2 * math.sqrt(2 * math.pi)
-                want = mul * norm * lpmv(mm, ll, ct)
-                assert abs(Q1 - want) <= 1e-12 * max(1.0, abs(want)), (ll, mm, ct)
+                    assert two or s["d1"] == 0.0
+                    t = s["d0"] * (c + s["d1"])
+                    dQ1, dQ2 = t * dQ1 + s["d0"] * dc * Q1 + s["g"] * dQ2, dQ1
+                    Q1, Q2 = t * Q1 + s["g"] * Q2, Q1
+                w = want(ll, mm, th)
+                assert abs(Q1 - w) <= 1e-12 * max(1.0, abs(w)), (ll, mm, th)
+                h = 1e-5
+                dw = (want(ll, mm, th + h) - want(ll, mm, th - h)) / (2 * h)
+                assert abs(dQ1 - dw) <= 1e-7 * max(1.0, abs(dw)), (ll, mm, th, dQ1, dw)
