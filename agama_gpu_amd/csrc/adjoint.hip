@@ -129,7 +129,7 @@ __device__ __forceinline__ void adjoint_interior(const MPK &P, int k, bool valid
     const AdjHook<SLAB> hook{gslab, sum, s, lam, mu, idl, wphi, wPr, wPt, wPf, lane};
     {
         PointOut o{};
-        interior<ORD_HESS, true>(P, slab, x, y, z, r2, r, lnr, k, o, 0.0, hook);
+        interior<ORD_HESS, true>(P, slab, x, y, z, r2, r, ir, lnr, k, o, 0.0, hook);
         if (valid) {
             gx[0] = wphi * o.g[0] - (o.h[0] * wfx + o.h[3] * wfy + o.h[5] * wfz);
             gx[1] = wphi * o.g[1] - (o.h[3] * wfx + o.h[1] * wfy + o.h[4] * wfz);

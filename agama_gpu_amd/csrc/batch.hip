@@ -51,7 +51,7 @@ __device__ __forceinline__ void batch_rows(const MPK &P, const double *tab_base,
         PointOut o{};
         Where w = classify(P, x, y, z);
         if (w.kind == 2)
-            interior<ORD, false>(P, tab_base + (int64_t)w.k * P.n_lm * 6, x, y, z, w.r2, w.r,
+            interior<ORD, false>(P, tab_base + (int64_t)w.k * P.n_lm * 6, x, y, z, w.r2, w.r, w.ir,
                                  w.lnr, w.k, o, 0.0);
         else
             extrap<ORD>(P, w, x, y, z, o, 0.0);

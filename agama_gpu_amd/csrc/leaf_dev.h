@@ -26,8 +26,8 @@ __device__ __forceinline__ void add_leaf(int type, const double *p, const void *
         PointOut o{};
         Where w = classify(mp, x, y, z);
         if (w.kind == 2)
-            interior<ORD, false>(mp, mp.poly + (int64_t)w.k * mp.n_lm * 6, x, y, z, w.r2, w.r, w.lnr,
-                                 w.k, o, 1.0 / (4.0 * M_PI * G_AGAMA_D));
+            interior<ORD, false>(mp, mp.poly + (int64_t)w.k * mp.n_lm * 6, x, y, z, w.r2, w.r,
+                                 w.ir, w.lnr, w.k, o, 1.0 / (4.0 * M_PI * G_AGAMA_D));
         else
             extrap<ORD>(mp, w, x, y, z, o, 1.0 / (4.0 * M_PI * G_AGAMA_D));
         if (RHO) e.rho = o.phi;

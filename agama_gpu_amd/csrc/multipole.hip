@@ -16,8 +16,8 @@ __global__ __launch_bounds__(kBlock) void mp_lane_kernel(MPK P, const double *__
         PointOut o{};
         Where w = classify(P, x, y, z);
         if (w.kind == 2)
-            interior<ORD, false>(P, P.poly + (int64_t)w.k * P.n_lm * 6, x, y, z, w.r2, w.r, w.lnr,
-                                 w.k, o, inv4piG);
+            interior<ORD, false>(P, P.poly + (int64_t)w.k * P.n_lm * 6, x, y, z, w.r2, w.r, w.ir,
+                                 w.lnr, w.k, o, inv4piG);
         else
             extrap<ORD>(P, w, x, y, z, o, inv4piG);
         store<ORD>(o, i, phi, force, deriv);
@@ -37,9 +37,10 @@ __global__ __launch_bounds__(kBinThreads) void mp_binned_kernel(
         if (b > 0) {
             const int k = b - 1;
             const double r2 = fma(x, x, fma(y, y, z * z));
-            const double r = sqrt(r2);
-            interior<ORD, true>(P, P.poly + (int64_t)k * P.n_lm * 6, x, y, z, r2, r, agx_log(r), k, o,
-                                inv4piG);
+            double r, ir;
+            sqrt_rsqrt(r2, r, ir);      // r >= r_min: r2 is far above 2^-767
+            interior<ORD, true>(P, P.poly + (int64_t)k * P.n_lm * 6, x, y, z, r2, r, ir, agx_log(r),
+                                k, o, inv4piG);
         } else {
             Where w = classify(P, x, y, z);
             extrap<ORD>(P, w, x, y, z, o, inv4piG);
@@ -82,7 +83,7 @@ __global__ __launch_bounds__(BLOCK) void mp_lds_kernel(MPK P, const double *__re
         Where w = classify(P, x, y, z);
         if (w.kind == 2)
             interior<ORD, false, NoHook, STRIDE2>(P, lds_tab + w.k * slab_len, x, y, z, w.r2,
-                                                  w.r, w.lnr, w.k, o, inv4piG);
+                                                  w.r, w.ir, w.lnr, w.k, o, inv4piG);
         else
             extrap<ORD>(P, w, x, y, z, o, inv4piG);
         store<ORD>(o, i, phi, force, deriv);
@@ -107,9 +108,10 @@ __global__ __launch_bounds__(kBlock) void mp_sorted_kernel(MPK P, const double *
         if (b > 0) {
             const int k = b - 1;
             const double r2 = fma(x, x, fma(y, y, z * z));
-            const double r = sqrt(r2);
-            interior<ORD, true>(P, P.poly + (int64_t)k * P.n_lm * 6, x, y, z, r2, r, agx_log(r), k, o,
-                                inv4piG);
+            double r, ir;
+            sqrt_rsqrt(r2, r, ir);      // r >= r_min: r2 is far above 2^-767
+            interior<ORD, true>(P, P.poly + (int64_t)k * P.n_lm * 6, x, y, z, r2, r, ir, agx_log(r),
+                                k, o, inv4piG);
         } else {
             const Where w = classify(P, x, y, z);
             extrap<ORD>(P, w, x, y, z, o, inv4piG);

@@ -117,8 +117,9 @@ struct NoHook {
 // UNI: the slab address is wave-uniform (scalar loads of coefficients).
 template <int ORD, bool UNI, class Hook = NoHook, bool STRIDE2 = true>
 __device__ __forceinline__ void interior(const MPK &P, const double *slab, double x, double y,
-                                         double z, double r2, double r, double lnr, int k,
-                                         PointOut &o, double inv4piG, const Hook &hook = Hook{})
+                                         double z, double r2, double r, double ir, double lnr,
+                                         int k, PointOut &o, double inv4piG,
+                                         const Hook &hook = Hook{})
 {
     constexpr bool R1 = ORD != ORD_POT;                      // d/d ln r
     constexpr bool D1 = ORD == ORD_GRAD || ORD == ORD_HESS;  // angular first derivatives
@@ -156,13 +157,14 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
     };
 
     // ---- angular basis ------------------------------------------------------
+    // R^2 below 2^-767 (R < 4e-116, far below 1e-10 r for any interior point)
+    // is taken as the axis: the pole branch, whose result does not depend on
+    // cos/sin(phi)
     const double R2 = fma(x, x, y * y);
-    const double R = sqrt(R2);
-    const double ir = rcp_nr(r);
+    double R = 0.0, iR = 0.0, cph = 1.0, sph = 0.0;
+    if (R2 >= 0x1p-767) { sqrt_rsqrt(R2, R, iR); cph = x * iR; sph = y * iR; }
     const double ct = z * ir, st = R * ir;
     const bool pole = st < 1.0e-10;
-    double cph = 1.0, sph = 0.0, iR = 0.0;
-    if (R > 1.0e-300) { iR = rcp_nr(R); cph = x * iR; sph = y * iR; }
     // 1/sin(theta) = r/R, needed inside the term loop only for the Hessian
     const double ist_loop = (ANG2 && !pole) ? r * iR : 0.0;
 
@@ -379,14 +381,19 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
 // Classification of one point: 0 origin, 1 inner law, 2 interior (k), 3 outer law.
 struct Where {
     int kind, k;
-    double r2, r, lnr;
+    double r2, r, ir, lnr;
 };
 
 __device__ __forceinline__ Where classify(const MPK &P, double x, double y, double z)
 {
     Where w;
     w.r2 = fma(x, x, fma(y, y, z * z));
-    w.r = sqrt(w.r2);
+    if (w.r2 >= 0x1p-767) {
+        sqrt_rsqrt(w.r2, w.r, w.ir);
+    } else {                  // r < 4e-116: the origin or the inner law
+        w.r = sqrt(w.r2);
+        w.ir = 0.0;
+    }
     w.k = 0;
     w.lnr = 0.0;
     if (!(w.r >= 1.0e-300)) { w.kind = 0; return w; }
