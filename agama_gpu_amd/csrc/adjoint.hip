@@ -82,7 +82,7 @@ struct AdjHook {
 // (LDS slab or memory); the invPhi0 gradient is accumulated per lane in ginv.
 // One pass: the Hessian-vector product's evaluation carries the table
 // gradient's scatter as its per-term hook.
-template <bool SLAB>
+template <bool SLAB, bool STRIDE2>
 __device__ __forceinline__ void adjoint_interior(const MPK &P, int k, bool valid, double x, double y,
                                                  double z, double wphi, const double wf[3],
                                                  double *gx, double *gslab, double &ginv_acc,
@@ -129,7 +129,8 @@ __device__ __forceinline__ void adjoint_interior(const MPK &P, int k, bool valid
     const AdjHook<SLAB> hook{gslab, sum, s, lam, mu, idl, wphi, wPr, wPt, wPf, lane};
     {
         PointOut o{};
-        interior<ORD_HESS, true>(P, slab, x, y, z, r2, r, ir, lnr, k, o, 0.0, hook);
+        interior<ORD_HESS, true, AdjHook<SLAB>, STRIDE2>(P, slab, x, y, z, r2, r, ir, lnr, k, o,
+                                                          0.0, hook);
         if (valid) {
             gx[0] = wphi * o.g[0] - (o.h[0] * wfx + o.h[3] * wfy + o.h[5] * wfz);
             gx[1] = wphi * o.g[1] - (o.h[3] * wfx + o.h[1] * wfy + o.h[4] * wfz);
@@ -206,7 +207,8 @@ __device__ __forceinline__ void flush_slab(double *slab, double *g, int cnt, int
 // The sorted order, 64 particles of one bin per wavefront and chunk.  Three
 // wavefronts per SIMD (at most 168 VGPRs: a few spill; measured faster than
 // two wavefronts without spills, 0.757 against 0.79 ms per gradient step).
-template <bool SLAB>
+// Without STRIDE2 the program must not recur by 2 (AGX_PROG_STRIDE1).
+template <bool SLAB, bool STRIDE2>
 __global__ __launch_bounds__(kAdjBlock, 3) void mp_adjoint_kernel(
     MPK P, const double *__restrict__ xyz, const double *__restrict__ gphi,
     const double *__restrict__ gforce, double *__restrict__ gxyz, double *__restrict__ gpoly,
@@ -234,8 +236,9 @@ __global__ __launch_bounds__(kAdjBlock, 3) void mp_adjoint_kernel(
         }
         double gx[3] = {0.0, 0.0, 0.0};
         if (b > 0)
-            adjoint_interior<SLAB>(P, b - 1, valid, x, y, z, wphi, wf, gx,
-                                   SLAB ? slab : gpoly + (int64_t)(b - 1) * slab_n, acc[0], lane);
+            adjoint_interior<SLAB, STRIDE2>(P, b - 1, valid, x, y, z, wphi, wf, gx,
+                                            SLAB ? slab : gpoly + (int64_t)(b - 1) * slab_n,
+                                            acc[0], lane);
         else
             adjoint_extrap(P, valid, x, y, z, wphi, wf, gx, acc + 1);
         if (valid && gxyz) { gxyz[3 * i] = gx[0]; gxyz[3 * i + 1] = gx[1]; gxyz[3 * i + 2] = gx[2]; }
@@ -262,6 +265,7 @@ extern "C" int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xy
     if (!mp->poly_km || !mp->program || !d_gpoly || !d_gscal) return AGX_E_NULL;
     if (mp->n_int < 1 || mp->n_int + 1 > kMaxBins || mp->n_lm < 1) return AGX_E_SIZE;
     if (mp->lmax > 32) return AGX_E_LMAX;
+    if (mp->prog_flags & ~AGX_PROG_STRIDE1) return AGX_E_WHAT;
     if (n == 0) return AGX_OK;
     if (!d_xyz || (!d_gphi && !d_gforce) || !d_work) return AGX_E_NULL;
     if (work_bytes < sort_work_size(n, kMaxBins)) return AGX_E_WORKSPACE;
@@ -279,11 +283,11 @@ extern "C" int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xy
     const int64_t wg_need = (chunks + run_len - 1) / run_len / (kAdjBlock / 64) + 1;
     const unsigned grid = (unsigned)(wg_need < 1024 ? wg_need : 1024);
     const size_t slab_bytes = (size_t)mp->n_lm * 6 * sizeof(double) * (kAdjBlock / 64);
-    if (slab_bytes <= (size_t)kAdjSlabLds)
-        hipLaunchKernelGGL(mp_adjoint_kernel<true>, dim3(grid), dim3(kAdjBlock), slab_bytes, st, P,
-                           d_xyz, d_gphi, d_gforce, d_gxyz, d_gpoly, d_gscal, W, run_len);
-    else
-        hipLaunchKernelGGL(mp_adjoint_kernel<false>, dim3(grid), dim3(kAdjBlock), 0, st, P, d_xyz,
-                           d_gphi, d_gforce, d_gxyz, d_gpoly, d_gscal, W, run_len);
+    const bool slab = slab_bytes <= (size_t)kAdjSlabLds;
+    const bool s2 = !(mp->prog_flags & AGX_PROG_STRIDE1);
+    auto kern = slab ? (s2 ? mp_adjoint_kernel<true, true> : mp_adjoint_kernel<true, false>)
+                     : (s2 ? mp_adjoint_kernel<false, true> : mp_adjoint_kernel<false, false>);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kAdjBlock), slab ? slab_bytes : 0, st, P, d_xyz,
+                       d_gphi, d_gforce, d_gxyz, d_gpoly, d_gscal, W, run_len);
     return hipGetLastError() == hipSuccess ? AGX_OK : AGX_E_LAUNCH;
 }
