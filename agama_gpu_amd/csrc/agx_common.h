@@ -108,6 +108,16 @@ __device__ __forceinline__ double sconst(double c)
     return c;
 }
 
+// A copy of v that the compiler cannot identify with v: expressions formed
+// from it are not merged with the same expressions of v formed earlier, so a
+// value needed only at the end of a kernel is recomputed there instead of
+// being held in registers across a loop.
+__device__ __forceinline__ double vopaque(double v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 __device__ __forceinline__ double agx_log(double x)
 {
     double m = __builtin_amdgcn_frexp_mant(x);     // [0.5, 1)

@@ -252,9 +252,8 @@ __global__ __launch_bounds__(kBlock) void cyl_adjoint_kernel(
             gxyz[3 * i + 1] = wphi * o.g[1] - (o.h[3] * wf[0] + o.h[1] * wf[1] + o.h[4] * wf[2]);
             gxyz[3 * i + 2] = wphi * o.g[2] - (o.h[5] * wf[0] + o.h[4] * wf[1] + o.h[2] * wf[2]);
         }
-        const double R = sqrt(fma(x, x, y * y));
-        const double iRs = 1.0 / P.Rscale;
-        const double lR = asinh(R * iRs), lz = asinh(z * iRs);
+        const CylCoords cc = cyl_coords(P, x, y, z);
+        const double R = cc.R, lR = cc.lR, lz = cc.lz;
         const bool outside = lR < cld(P.lR, 0) || lR > cld(P.lR, P.nR - 1) ||
                              lz < cld(P.lz, 0) || lz > cld(P.lz, P.nz - 1);
         if (valid && !outside) cyl_adjoint_inner<false>(P, x, y, z, R, lR, lz, wphi, wf, gnodes, 0, 0);
@@ -269,9 +268,8 @@ struct CylCellKey {
     const double *xyz;
     __device__ int operator()(int64_t i) const
     {
-        const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
-        const double iRs = 1.0 / P.Rscale;
-        const double lR = asinh(sqrt(fma(x, x, y * y)) * iRs), lz = asinh(z * iRs);
+        const CylCoords c = cyl_coords(P, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+        const double lR = c.lR, lz = c.lz;
         if (lR < cld(P.lR, 0) || lR > cld(P.lR, P.nR - 1) || lz < cld(P.lz, 0) ||
             lz > cld(P.lz, P.nz - 1))
             return 0;
@@ -308,11 +306,10 @@ __global__ __launch_bounds__(kBlock) void cyl_adjoint_sorted_kernel(
                 gxyz[3 * i + 2] = wphi * o.g[2] - (o.h[5] * wf[0] + o.h[4] * wf[1] + o.h[2] * wf[2]);
             }
         }
-        const double R = sqrt(fma(x, x, y * y));
+        const CylCoords cc = cyl_coords(P, x, y, z);
+        const double R = cc.R;
         if (b > 0) {
-            const double iRs = 1.0 / P.Rscale;
-            const double lR = asinh(R * iRs), lz = asinh(z * iRs);
-            cyl_adjoint_inner<true>(P, x, y, z, R, lR, lz, wphi, wf, gnodes, iR, iz);
+            cyl_adjoint_inner<true>(P, x, y, z, R, cc.lR, cc.lz, wphi, wf, gnodes, iR, iz);
         } else {
             cyl_adjoint_outer(P, valid, x, y, z, R, wphi, wf, gW);
         }

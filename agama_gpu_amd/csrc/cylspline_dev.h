@@ -27,7 +27,32 @@ struct CSK {
     const double *W;              // [(lmax_o+1)^2] outer PowerLaw, index l(l+1)+m
     int nR, nz, n_harm, log_scaling, lmax_o, mmax_o;
     double Rscale, r0;
+    double lnRs;                  // ln Rscale
 };
+
+// The point's cylindrical radius and scaled coordinates.  asinh(X / Rscale)
+// = ln(X + sqrt(X^2 + Rscale^2)) - ln Rscale, whose derivative in X,
+// 1/sqrt(X^2 + Rscale^2), comes from the same square root; lR and |lz| are
+// clamped at 0 (rounding near the axis and the midplane must not move a point
+// below a grid that starts at 0).  R^2 below 2^-767 is the axis: R = 1/R = 0.
+struct CylCoords {
+    double R, iR, lR, lz, jR, jz;
+};
+
+__device__ __forceinline__ CylCoords cyl_coords(const CSK &P, double x, double y, double z)
+{
+    CylCoords c;
+    const double R2 = fma(x, x, y * y), Rs2 = P.Rscale * P.Rscale;
+    c.R = 0.0;
+    c.iR = 0.0;
+    if (R2 >= 0x1p-767) sqrt_rsqrt(R2, c.R, c.iR);
+    double qR, qz;
+    sqrt_rsqrt(R2 + Rs2, qR, c.jR);
+    sqrt_rsqrt(fma(z, z, Rs2), qz, c.jz);
+    c.lR = fmax(agx_log(c.R + qR) - P.lnRs, 0.0);
+    c.lz = copysign(fmax(agx_log(fabs(z) + qz) - P.lnRs, 0.0), z);
+    return c;
+}
 
 // cell index i with grid[i] <= x < grid[i+1], clamped to [0, n-2]
 __device__ __forceinline__ int cell_search(double x, const double *g, int n)
@@ -90,18 +115,26 @@ constexpr OuterRecurrence make_outer_recurrence()
 
 __constant__ const OuterRecurrence kOuterRec = make_outer_recurrence();
 
-// Outer PowerLaw multipole (lmax_o <= 8): spherical derivatives.
+// Outer PowerLaw multipole (lmax_o <= 8): spherical derivatives.  R and 1/R
+// as cyl_coords gives them.
 template <bool G, bool H, bool RHO>
 __device__ __forceinline__ void cyl_outer(const CSK &P, double x, double y, double z, double R,
-                                          PointOut &o)
+                                          double iR, PointOut &o)
 {
-    const double r2 = fma(R, R, z * z), r = sqrt(r2), ir = 1.0 / r;
     if (RHO) { o.phi = 0.0; return; }   // a harmonic expansion: zero density
+    const double r2 = fma(R, R, z * z);
+    double r, ir;
+    if (r2 >= 0x1p-767) {
+        sqrt_rsqrt(r2, r, ir);
+    } else {
+        r = sqrt(r2);
+        ir = 1.0 / r;
+    }
     const double ct = z * ir, st = R * ir;
     const bool pole = st < 1.0e-10;
     double cph = 1.0, sph = 0.0;
-    if (R > 1.0e-300) { cph = x / R; sph = y / R; }
-    const double ist = pole ? 0.0 : 1.0 / st;
+    if (R > 0.0) { cph = x * iR; sph = y * iR; }
+    const double ist = pole ? 0.0 : r * iR;
     const double q = P.r0 * ir;
     double Sv = 0, Sr = 0, Srr = 0, St = 0, Stt = 0, Srt = 0, Sf = 0, Sff = 0, Srf = 0, Stf = 0;
     double Sfn = 0, Sfp = 0;
@@ -178,21 +211,20 @@ __device__ __forceinline__ void cylspline_point(const CSK &P, double x, double y
 {
     constexpr bool G = ORD == ORD_GRAD || ORD == ORD_HESS;
     constexpr bool H = ORD == ORD_HESS || ORD == ORD_DENS;   // density from the trace
-    const double R2 = fma(x, x, y * y), R = sqrt(R2);
-    const double iRs = 1.0 / P.Rscale;
-    const double lR = asinh(R * iRs), lz = asinh(z * iRs);
+    const CylCoords cc = cyl_coords(P, x, y, z);
+    const double R = cc.R, lR = cc.lR, lz = cc.lz;
     if (lR < cld(P.lR, 0) || lR > cld(P.lR, P.nR - 1) || lz < cld(P.lz, 0) ||
         lz > cld(P.lz, P.nz - 1)) {
-        cyl_outer<G || H, H, ORD == ORD_DENS>(P, x, y, z, R, o);
+        cyl_outer<G || H, H, ORD == ORD_DENS>(P, x, y, z, R, cc.iR, o);
         return;
     }
     const int iR = cell_search(lR, P.lR, P.nR), iz = cell_search(lz, P.lz, P.nz);
     const double lR0 = cld(P.lR, iR), lR1 = cld(P.lR, iR + 1);
     const double lz0 = cld(P.lz, iz), lz1 = cld(P.lz, iz + 1);
     const double hR = lR1 - lR0, hz = lz1 - lz0;
-    const double tR = (lR - lR0) / hR, tz = (lz - lz0) / hz;
+    const double tR = (lR - lR0) * rcp_nr(hR), tz = (lz - lz0) * rcp_nr(hz);
     double cph = 1.0, sph = 0.0;
-    if (R > 1.0e-300) { cph = x / R; sph = y / R; }
+    if (R > 0.0) { cph = x * cc.iR; sph = y * cc.iR; }
 
     // Fourier sums of the (scaled) harmonics and their lR, lz, phi derivatives
     double S = P.log_scaling ? 1.0 : 0.0, SR = 0, SZ = 0, SRR = 0, SRZ = 0, SZZ = 0;
@@ -278,8 +310,13 @@ __device__ __forceinline__ void cylspline_point(const CSK &P, double x, double y
     }
     if (ORD == ORD_POT) { o.phi = Pv; return; }
     // ---- (lR, lz) -> (R, z): dlX/dX = 1/sqrt(X^2 + Rscale^2) -----------------
-    const double Rs2 = P.Rscale * P.Rscale;
-    const double jR = 1.0 / sqrt(R2 + Rs2), jz = 1.0 / sqrt(fma(z, z, Rs2));
+    // the Jacobian and 1/R formed again here (from opaque copies of R and z)
+    // rather than held in registers across the harmonic loop
+    const double Rq = vopaque(R), zq = vopaque(z), Rs2 = P.Rscale * P.Rscale;
+    double qq, jR, jz;
+    sqrt_rsqrt(fma(Rq, Rq, Rs2), qq, jR);
+    sqrt_rsqrt(fma(zq, zq, Rs2), qq, jz);
+    const double invR = R > 0.0 ? rcp_nr(Rq) : 0.0;
     const double PhR = PR * jR, PhZ = PZ * jz;
     double HRR = 0, HZZ = 0, HRZ = 0, HRp = 0, HZp = 0;
     if (H) {
@@ -291,30 +328,29 @@ __device__ __forceinline__ void cylspline_point(const CSK &P, double x, double y
     }
     if (ORD == ORD_DENS) {
         // Laplacian in cylindrical coordinates: Phi_RR + Phi_R/R + Phi_pp/R^2 + Phi_zz
-        const double lap = (R > 1.0e-30) ? HRR + PhR / R + Ppp / R2 + HZZ : 2.0 * HRR + HZZ;
+        const double lap = (R > 1.0e-30) ? HRR + (PhR + Ppp * invR) * invR + HZZ : 2.0 * HRR + HZZ;
         o.phi = lap * (1.0 / (4.0 * M_PI * G_AGAMA_D));
         return;
     }
     o.phi = Pv;
     // ---- (R, z, phi) -> Cartesian ---------------------------------------------
     if (R > 1.0e-30) {
-        const double iR = 1.0 / R;
-        o.g[0] = PhR * cph - Pp * sph * iR;
-        o.g[1] = PhR * sph + Pp * cph * iR;
+        o.g[0] = PhR * cph - Pp * sph * invR;
+        o.g[1] = PhR * sph + Pp * cph * invR;
     } else {
         o.g[0] = 0.0; o.g[1] = 0.0;
     }
     o.g[2] = PhZ;
     if (ORD != ORD_HESS) return;
     if (R > 1.0e-30) {
-        const double iR = 1.0 / R, iR2 = iR * iR;
+        const double iR2 = invR * invR;
         const double c2 = cph * cph, s2 = sph * sph, cs = cph * sph;
         // d/dx = cos d/dR - sin/R d/dphi, d/dy = sin d/dR + cos/R d/dphi
-        o.h[0] = HRR * c2 - 2.0 * HRp * cs * iR + (PhR * iR) * s2 + (2.0 * Pp * iR2) * cs + Ppp * s2 * iR2;
-        o.h[1] = HRR * s2 + 2.0 * HRp * cs * iR + (PhR * iR) * c2 - (2.0 * Pp * iR2) * cs + Ppp * c2 * iR2;
-        o.h[3] = HRR * cs + HRp * (c2 - s2) * iR - (PhR * iR) * cs - Pp * (c2 - s2) * iR2 - Ppp * cs * iR2;
-        o.h[4] = HRZ * sph + HZp * cph * iR;
-        o.h[5] = HRZ * cph - HZp * sph * iR;
+        o.h[0] = HRR * c2 - 2.0 * HRp * cs * invR + (PhR * invR) * s2 + (2.0 * Pp * iR2) * cs + Ppp * s2 * iR2;
+        o.h[1] = HRR * s2 + 2.0 * HRp * cs * invR + (PhR * invR) * c2 - (2.0 * Pp * iR2) * cs + Ppp * c2 * iR2;
+        o.h[3] = HRR * cs + HRp * (c2 - s2) * invR - (PhR * invR) * cs - Pp * (c2 - s2) * iR2 - Ppp * cs * iR2;
+        o.h[4] = HRZ * sph + HZp * cph * invR;
+        o.h[5] = HRZ * cph - HZp * sph * invR;
     } else {
         o.h[0] = HRR; o.h[1] = HRR; o.h[3] = 0.0; o.h[4] = 0.0; o.h[5] = 0.0;
     }
@@ -329,6 +365,7 @@ inline CSK make_cyl_args(const agx_cylspline *cs)
     P.nR = cs->nR; P.nz = cs->nz; P.n_harm = cs->n_harm; P.log_scaling = cs->log_scaling;
     P.lmax_o = cs->lmax_outer; P.mmax_o = cs->mmax_outer;
     P.Rscale = cs->Rscale; P.r0 = cs->r0_outer;
+    P.lnRs = std::log(cs->Rscale);
     return P;
 }
 
