@@ -47,20 +47,105 @@ __global__ __launch_bounds__(kBlock) void composite_kernel(CompositeArgs A,
     });
 }
 
-static int check_components(const agx_component *comps, int n_comps, CompositeArgs &A)
+// Sums whose leaves are all analytic types, which are bound by HBM.  Each
+// workgroup moves tiles of kBlock particles through LDS, so that the global
+// reads of the positions and the writes of force and deriv are contiguous
+// 8-byte accesses across the lanes (instead of one access per component at a
+// 24- or 48-byte stride per lane), and the next tile's positions are read
+// into registers while the current tile computes.  No expansion code is
+// compiled in.
+template <int ORD>
+__global__ __launch_bounds__(kBlock) void analytic_kernel(CompositeArgs A,
+                                                          const double *__restrict__ xyz,
+                                                          int64_t n, double *__restrict__ phi,
+                                                          double *__restrict__ force,
+                                                          double *__restrict__ deriv)
+{
+    constexpr bool G = ORD == ORD_GRAD || ORD == ORD_HESS;
+    constexpr bool H = ORD == ORD_HESS;
+    __shared__ double L[(H ? 6 : 3) * kBlock];
+    const int t = threadIdx.x;
+    const int64_t tstride = (int64_t)gridDim.x * kBlock;
+    int64_t base = (int64_t)blockIdx.x * kBlock;
+    double pre[3];
+    auto fetch = [&](int64_t b) {          // the 3 kBlock position doubles from particle b on
+        const int64_t avail = 3 * (n - b);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int j = t + k * kBlock;
+            pre[k] = (j < avail) ? xyz[3 * b + j] : 0.0;
+        }
+    };
+    // a contiguous run of m doubles per particle from LDS to out + m * base
+    auto flush = [&](double *out, int m, int64_t b) {
+        const int64_t avail = (int64_t)m * (n - b);
+        for (int j = t; j < m * kBlock; j += kBlock)
+            if (j < avail) out[(int64_t)m * b + j] = L[j];
+    };
+    if (base < n) fetch(base);
+    for (; base < n; base += tstride) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) L[t + k * kBlock] = pre[k];
+        __syncthreads();
+        const double x0 = L[3 * t], y0 = L[3 * t + 1], z0 = L[3 * t + 2];
+        if (base + tstride < n) fetch(base + tstride);
+        const int64_t i = base + t;
+        double P = 0.0, g[3] = {0.0, 0.0, 0.0}, h[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (i < n) {
+            for (int c = 0; c < A.n; ++c) {
+                const agx_component &C = A.c[c];
+                add_leaf<ORD, false>(C.type, C.p, C.table, C.ampl, C.scale, C.center, x0, y0, z0,
+                                     P, g, h);
+            }
+            if (phi) phi[i] = P;
+        }
+        __syncthreads();                   // the positions in L are read
+        if (G && force) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) L[3 * t + k] = -g[k];
+            __syncthreads();
+            flush(force, 3, base);
+            __syncthreads();
+        }
+        if (H && deriv) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) L[6 * t + k] = -h[k];
+            __syncthreads();
+            flush(deriv, 6, base);
+            __syncthreads();
+        }
+    }
+}
+
+static int check_components(const agx_component *comps, int n_comps, CompositeArgs &A,
+                            bool &any_expansion)
 {
     if (n_comps < 1 || n_comps > kMaxComponents) return AGX_E_SIZE;
     if (!comps) return AGX_E_NULL;
     A.n = n_comps;
+    any_expansion = false;
     for (int c = 0; c < n_comps; ++c) {
         const agx_component &C = comps[c];
         const bool expansion = C.type == AGX_T_MULTIPOLE || C.type == AGX_T_CYLSPLINE;
         const bool analytic = C.type >= AGX_T_NFW && C.type <= AGX_T_UNIFORM_ACCEL;
         if (!expansion && !analytic) return AGX_E_TYPE;
         if (expansion && !C.table) return AGX_E_NULL;
+        any_expansion = any_expansion || expansion;
         A.c[c] = C;
     }
     return AGX_OK;
+}
+
+template <int ORD>
+static void launch_composite(const CompositeArgs &A, bool expansion, const double *xyz, int64_t n,
+                             double *phi, double *force, double *deriv, hipStream_t s)
+{
+    if (expansion)
+        hipLaunchKernelGGL(composite_kernel<ORD>, dim3(grid_for(n)), dim3(kBlock), 0, s, A, xyz,
+                           n, phi, force, deriv);
+    else
+        hipLaunchKernelGGL(analytic_kernel<ORD>, dim3(grid_for(n)), dim3(kBlock), 0, s, A, xyz, n,
+                           phi, force, deriv);
 }
 
 }  // namespace agx
@@ -72,7 +157,8 @@ extern "C" int agx_composite_eval(const agx_component *comps, int n_comps, const
                                   uint32_t what, void *stream)
 {
     CompositeArgs A;
-    int rc = check_components(comps, n_comps, A);
+    bool exp_leaves;
+    int rc = check_components(comps, n_comps, A, exp_leaves);
     if (rc) return rc;
     if (n < 0) return AGX_E_SIZE;
     if (what == 0 || (what & ~(AGX_PHI | AGX_FORCE | AGX_DERIV))) return AGX_E_WHAT;
@@ -84,13 +170,12 @@ extern "C" int agx_composite_eval(const agx_component *comps, int n_comps, const
     double *force = (what & AGX_FORCE) ? d_force : nullptr;
     double *deriv = (what & AGX_DERIV) ? d_deriv : nullptr;
     hipStream_t s = (hipStream_t)stream;
-    dim3 grid(grid_for(n)), block(kBlock);
     if (deriv)
-        hipLaunchKernelGGL(composite_kernel<ORD_HESS>, grid, block, 0, s, A, d_xyz, n, phi, force, deriv);
+        launch_composite<ORD_HESS>(A, exp_leaves, d_xyz, n, phi, force, deriv, s);
     else if (force)
-        hipLaunchKernelGGL(composite_kernel<ORD_GRAD>, grid, block, 0, s, A, d_xyz, n, phi, force, nullptr);
+        launch_composite<ORD_GRAD>(A, exp_leaves, d_xyz, n, phi, force, nullptr, s);
     else
-        hipLaunchKernelGGL(composite_kernel<ORD_POT>, grid, block, 0, s, A, d_xyz, n, phi, nullptr, nullptr);
+        launch_composite<ORD_POT>(A, exp_leaves, d_xyz, n, phi, nullptr, nullptr, s);
     return hipGetLastError() == hipSuccess ? AGX_OK : AGX_E_LAUNCH;
 }
 
@@ -98,13 +183,14 @@ extern "C" int agx_composite_density(const agx_component *comps, int n_comps, co
                                      int64_t n, double *d_rho, void *stream)
 {
     CompositeArgs A;
-    int rc = check_components(comps, n_comps, A);
+    bool exp_leaves;
+    int rc = check_components(comps, n_comps, A, exp_leaves);
     if (rc) return rc;
     if (n < 0) return AGX_E_SIZE;
     if (n == 0) return AGX_OK;
     if (!d_xyz || !d_rho) return AGX_E_NULL;
-    hipLaunchKernelGGL(composite_kernel<ORD_DENS>, dim3(grid_for(n)), dim3(kBlock), 0,
-                       (hipStream_t)stream, A, d_xyz, n, d_rho, nullptr, nullptr);
+    launch_composite<ORD_DENS>(A, exp_leaves, d_xyz, n, d_rho, nullptr, nullptr,
+                               (hipStream_t)stream);
     return hipGetLastError() == hipSuccess ? AGX_OK : AGX_E_LAUNCH;
 }
 

@@ -11,7 +11,9 @@
 
 namespace agx {
 
-template <int ORD>
+// EXP = false: the caller guarantees an analytic type (no expansion code is
+// compiled in, which keeps analytic-only kernels small in registers).
+template <int ORD, bool EXP = true>
 __device__ __forceinline__ void add_leaf(int type, const double *p, const void *table, double ampl,
                                          double sg, const double *center, double x0, double y0,
                                          double z0, double &P, double g[3], double h[6])
@@ -21,7 +23,7 @@ __device__ __forceinline__ void add_leaf(int type, const double *p, const void *
     constexpr bool RHO = ORD == ORD_DENS;
     const double x = sg * (x0 - center[0]), y = sg * (y0 - center[1]), z = sg * (z0 - center[2]);
     Eval3 e;
-    if (type == AGX_T_MULTIPOLE) {
+    if (EXP && type == AGX_T_MULTIPOLE) {
         const MPK mp = with_law(load_desc<MPK>(table));
         PointOut o{};
         Where w = classify(mp, x, y, z);
@@ -34,7 +36,7 @@ __device__ __forceinline__ void add_leaf(int type, const double *p, const void *
         else e.phi = o.phi;
         for (int k = 0; k < 3; ++k) e.g[k] = o.g[k];
         for (int k = 0; k < 6; ++k) e.h[k] = o.h[k];
-    } else if (type == AGX_T_CYLSPLINE) {
+    } else if (EXP && type == AGX_T_CYLSPLINE) {
         const CSK cs = load_desc<CSK>(table);
         PointOut o{};
         cylspline_point<ORD>(cs, x, y, z, o);
