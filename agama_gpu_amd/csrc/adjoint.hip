@@ -265,7 +265,7 @@ extern "C" int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xy
     if (!mp->poly_km || !mp->program || !d_gpoly || !d_gscal) return AGX_E_NULL;
     if (mp->n_int < 1 || mp->n_int + 1 > kMaxBins || mp->n_lm < 1) return AGX_E_SIZE;
     if (mp->lmax > 32) return AGX_E_LMAX;
-    if (mp->prog_flags & ~AGX_PROG_STRIDE1) return AGX_E_WHAT;
+    if (mp->prog_flags & ~(AGX_PROG_STRIDE1 | AGX_PROG_COS_ONLY)) return AGX_E_WHAT;
     if (n == 0) return AGX_OK;
     if (!d_xyz || (!d_gphi && !d_gforce) || !d_work) return AGX_E_NULL;
     if (work_bytes < sort_work_size(n, kMaxBins)) return AGX_E_WORKSPACE;

@@ -115,7 +115,7 @@ struct NoHook {
 
 // Interior evaluation on radial interval k (slab = poly + k*n_lm*6).
 // UNI: the slab address is wave-uniform (scalar loads of coefficients).
-template <int ORD, bool UNI, class Hook = NoHook, bool STRIDE2 = true>
+template <int ORD, bool UNI, class Hook = NoHook, bool STRIDE2 = true, bool SIN = true>
 __device__ __forceinline__ void interior(const MPK &P, const double *slab, double x, double y,
                                          double z, double r2, double r, double ir, double lnr,
                                          int k, PointOut &o, double inv4piG,
@@ -242,7 +242,7 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
                     add_term(ac, Q, dQ, d2Q, ll1, Vc, Rc, Tc, RRc, TTc, RTc, Lc);
                 }
             }
-            if (os >= 0) {
+            if (SIN && os >= 0) {
                 double as[6];
                 load6(os, as);
                 add_term(as, Q, dQ, d2Q, ll1, Vs, Rs, Ts, RRs, TTs, RTs, Ls);
@@ -297,29 +297,35 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
             terms(r1, QB, dQB, cB);
         }
         // ---- apply cos/sin(m phi) to the group sums ------------------------
-        Sv = fma(cm, Vc, fma(sm, Vs, Sv));
-        if (R1) Sr = fma(cm, Rc, fma(sm, Rs, Sr));
+        // acc + cos C + sin S, and cos S - sin C; without SIN (no sin terms in
+        // the program) the S parts are zero and left out
+        auto comb = [&](double c, double sv, double acc) {
+            return SIN ? fma(cm, c, fma(sm, sv, acc)) : fma(cm, c, acc);
+        };
+        auto rot = [&](double sv, double c) { return SIN ? fma(cm, sv, -sm * c) : -sm * c; };
+        Sv = comb(Vc, Vs, Sv);
+        if (R1) Sr = comb(Rc, Rs, Sr);
         if (D1) {
-            St = fma(cm, Tc, fma(sm, Ts, St));
+            St = comb(Tc, Ts, St);
             if (m > 0) {
                 const double fm = (double)m;
-                const double Fv = fm * fma(cm, Vs, -sm * Vc);     // dPhi/dphi (group)
+                const double Fv = fm * rot(Vs, Vc);                 // dPhi/dphi (group)
                 Sfn += Fv;
-                if (m == 1) Sfp += fma(cm, Ts, -sm * Tc);         // pole limit of Y/sin
+                if (m == 1) Sfp += rot(Ts, Tc);                     // pole limit of Y/sin
                 if (ANG2) {
                     Sf += Fv;
-                    Sff -= fm * fm * fma(cm, Vc, sm * Vs);
-                    Srf += fm * fma(cm, Rs, -sm * Rc);
-                    Stf += fm * fma(cm, Ts, -sm * Tc);
+                    Sff -= fm * fm * (SIN ? fma(cm, Vc, sm * Vs) : cm * Vc);
+                    Srf += fm * rot(Rs, Rc);
+                    Stf += fm * rot(Ts, Tc);
                 }
             }
         }
-        if (D2) Srr = fma(cm, RRc, fma(sm, RRs, Srr));
+        if (D2) Srr = comb(RRc, RRs, Srr);
         if (ANG2) {
-            Stt = fma(cm, TTc, fma(sm, TTs, Stt));
-            Srt = fma(cm, RTc, fma(sm, RTs, Srt));
+            Stt = comb(TTc, TTs, Stt);
+            Srt = comb(RTc, RTs, Srt);
         }
-        if (ORD == ORD_DENS) SL = fma(cm, Lc, fma(sm, Ls, SL));
+        if (ORD == ORD_DENS) SL = comb(Lc, Ls, SL);
         gr = gn;
     }
 

@@ -71,13 +71,17 @@ enum {
  *                       (l, -m) term in a poly_km slab, -1 if not active)
  *       last            all zero
  *             step_begin/step_end index the step records from 0.
- *   prog_flags : 0, or AGX_PROG_STRIDE1 when every group of `program` with
- *             more than one step has stride 1; the LDS-staged kernels then
- *             run a variant without the stride-2 arithmetic.  0 is always
- *             correct; setting the flag for a program that does step by 2
- *             gives wrong results.
+ *   prog_flags : an OR of
+ *             AGX_PROG_STRIDE1  every group of `program` with more than one
+ *                               step has stride 1;
+ *             AGX_PROG_COS_ONLY no step of `program` has a sin term (every
+ *                               sin_off is -1: tables with m >= 0 only);
+ *             the kernels then run variants without the stride-2 arithmetic
+ *             or the sin sums.  0 is always correct; a flag set for a program
+ *             that does not satisfy it gives wrong results.
  * ---------------------------------------------------------------------- */
 #define AGX_PROG_STRIDE1 1u
+#define AGX_PROG_COS_ONLY 2u
 
 typedef struct agx_multipole {
     const double *poly_km;
