@@ -154,8 +154,8 @@ _SIGNATURES = {
                                C.c_int64, C.c_double, C.c_int32, C.c_void_p]),
     "agx_multipole_batch_eval": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p,
                                            C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
-                                           C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
-                                           C.c_void_p]),
+                                           C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
+                                           C.c_void_p, C.c_void_p]),
 }
 
 

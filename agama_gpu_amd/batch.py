@@ -71,6 +71,10 @@ class MultipoleEnsembleGPU:
             _lib.check(L.agx_multipole_pack(C.byref(p.descriptor), buf, size), "agx_multipole_pack")
             img[j, :size] = np.frombuffer(buf, np.uint8)
         self._desc = torch.as_tensor(img.ravel(), device=self.device)
+        # the program flags every potential satisfies (selects the kernel variant)
+        self._prog_flags = 0xFFFFFFFF
+        for p in self.potentials:
+            self._prog_flags &= int(p.descriptor.prog_flags)
         path = np.array([c[0] for c in self.choices], np.int32)
         chunk = np.array([c[1] for c in self.choices], np.int32)
         cost = np.array([_cost_per_particle(p) for p in self.potentials], np.float64)
@@ -106,7 +110,7 @@ class MultipoleEnsembleGPU:
         rc = _lib.load().agx_multipole_batch_eval(
             self._desc.data_ptr(), self._stride, len(self.potentials), self._items.data_ptr(),
             len(self.items), xyz.data_ptr(), self.n_rows, ptr(phi), ptr(force), ptr(deriv), what,
-            self._queue.data_ptr(), _lib.stream_handle(self.device))
+            self._prog_flags, self._queue.data_ptr(), _lib.stream_handle(self.device))
         _lib.check(rc, "agx_multipole_batch_eval")
         return phi, force, deriv
 

@@ -304,7 +304,9 @@ int64_t agx_cylspline_adjoint_work_size(const agx_cylspline *cs, int64_t n);
  * memset) and uses as the work counter; concurrent calls need separate
  * queues.  Items outside [0, n_rows) or naming no potential are skipped; an
  * AGX_BATCH_LDS item whose table is larger than
- * agx_batch_lds_table_doubles() doubles is evaluated as AGX_BATCH_LANE.  No
+ * agx_batch_lds_table_doubles() doubles is evaluated as AGX_BATCH_LANE.
+ * prog_flags: the AGX_PROG_* flags that hold for every potential's program
+ * (the AND of their agx_multipole.prog_flags; 0 is always correct).  No
  * reference counterpart (the reference launches once per potential,
  * gpu_potential.py:757-793).
  * ------------------------------------------------------------------------- */
@@ -327,7 +329,7 @@ int agx_multipole_batch_eval(const void *d_desc, int64_t desc_stride, int32_t n_
                              const agx_batch_item *d_items, int64_t n_items,
                              const double *d_xyz, int64_t n_rows, double *d_phi,
                              double *d_force, double *d_deriv, uint32_t what,
-                             int32_t *d_queue, void *stream);
+                             uint32_t prog_flags, int32_t *d_queue, void *stream);
 
 /* ---------------------------------------------------------------------------
  * Leapfrog (kick-drift-kick) of N test particles through a time-dependent
