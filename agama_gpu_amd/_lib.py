@@ -76,6 +76,7 @@ AGX_PHI, AGX_FORCE, AGX_DERIV = 1, 2, 4
 AGX_WORK_SORTED = 1
 AGX_PROG_STRIDE1 = 1
 AGX_PROG_COS_ONLY = 2
+AGX_PROG_STRIDE2 = 4
 AGX_OPT_MP_PATH, AGX_OPT_MP_TILE, AGX_OPT_MP_BIN_MIN_N, AGX_OPT_MP_BIN_MIN_TERMS = 1, 2, 3, 4
 AGX_OPT_MP_LDS_GRID = 6
 AGX_OPT_MP_BIN_GRID = 7

@@ -82,7 +82,7 @@ struct AdjHook {
 // (LDS slab or memory); the invPhi0 gradient is accumulated per lane in ginv.
 // One pass: the Hessian-vector product's evaluation carries the table
 // gradient's scatter as its per-term hook.
-template <bool SLAB, bool STRIDE2>
+template <bool SLAB, int STRIDE>
 __device__ __forceinline__ void adjoint_interior(const MPK &P, int k, bool valid, double x, double y,
                                                  double z, double wphi, const double wf[3],
                                                  double *gx, double *gslab, double &ginv_acc,
@@ -129,7 +129,7 @@ __device__ __forceinline__ void adjoint_interior(const MPK &P, int k, bool valid
     const AdjHook<SLAB> hook{gslab, sum, s, lam, mu, idl, wphi, wPr, wPt, wPf, lane};
     {
         PointOut o{};
-        interior<ORD_HESS, true, AdjHook<SLAB>, STRIDE2>(P, slab, x, y, z, r2, r, ir, lnr, k, o,
+        interior<ORD_HESS, true, AdjHook<SLAB>, STRIDE>(P, slab, x, y, z, r2, r, ir, lnr, k, o,
                                                           0.0, hook);
         if (valid) {
             gx[0] = wphi * o.g[0] - (o.h[0] * wfx + o.h[3] * wfy + o.h[5] * wfz);
@@ -207,8 +207,8 @@ __device__ __forceinline__ void flush_slab(double *slab, double *g, int cnt, int
 // The sorted order, 64 particles of one bin per wavefront and chunk.  Three
 // wavefronts per SIMD (at most 168 VGPRs: a few spill; measured faster than
 // two wavefronts without spills, 0.757 against 0.79 ms per gradient step).
-// Without STRIDE2 the program must not recur by 2 (AGX_PROG_STRIDE1).
-template <bool SLAB, bool STRIDE2>
+// STRIDE 1: the program recurs by 1 only (AGX_PROG_STRIDE1); 0: any program.
+template <bool SLAB, int STRIDE>
 __global__ __launch_bounds__(kAdjBlock, 3) void mp_adjoint_kernel(
     MPK P, const double *__restrict__ xyz, const double *__restrict__ gphi,
     const double *__restrict__ gforce, double *__restrict__ gxyz, double *__restrict__ gpoly,
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(kAdjBlock, 3) void mp_adjoint_kernel(
         }
         double gx[3] = {0.0, 0.0, 0.0};
         if (b > 0)
-            adjoint_interior<SLAB, STRIDE2>(P, b - 1, valid, x, y, z, wphi, wf, gx,
+            adjoint_interior<SLAB, STRIDE>(P, b - 1, valid, x, y, z, wphi, wf, gx,
                                             SLAB ? slab : gpoly + (int64_t)(b - 1) * slab_n,
                                             acc[0], lane);
         else
@@ -265,7 +265,7 @@ extern "C" int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xy
     if (!mp->poly_km || !mp->program || !d_gpoly || !d_gscal) return AGX_E_NULL;
     if (mp->n_int < 1 || mp->n_int + 1 > kMaxBins || mp->n_lm < 1) return AGX_E_SIZE;
     if (mp->lmax > 32) return AGX_E_LMAX;
-    if (mp->prog_flags & ~(AGX_PROG_STRIDE1 | AGX_PROG_COS_ONLY)) return AGX_E_WHAT;
+    if (mp->prog_flags & ~kProgFlags) return AGX_E_WHAT;
     if (n == 0) return AGX_OK;
     if (!d_xyz || (!d_gphi && !d_gforce) || !d_work) return AGX_E_NULL;
     if (work_bytes < sort_work_size(n, kMaxBins)) return AGX_E_WORKSPACE;
@@ -284,9 +284,9 @@ extern "C" int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xy
     const unsigned grid = (unsigned)(wg_need < 1024 ? wg_need : 1024);
     const size_t slab_bytes = (size_t)mp->n_lm * 6 * sizeof(double) * (kAdjBlock / 64);
     const bool slab = slab_bytes <= (size_t)kAdjSlabLds;
-    const bool s2 = !(mp->prog_flags & AGX_PROG_STRIDE1);
-    auto kern = slab ? (s2 ? mp_adjoint_kernel<true, true> : mp_adjoint_kernel<true, false>)
-                     : (s2 ? mp_adjoint_kernel<false, true> : mp_adjoint_kernel<false, false>);
+    const bool s1 = mp->prog_flags & AGX_PROG_STRIDE1;
+    auto kern = slab ? (s1 ? mp_adjoint_kernel<true, 1> : mp_adjoint_kernel<true, 0>)
+                     : (s1 ? mp_adjoint_kernel<false, 1> : mp_adjoint_kernel<false, 0>);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kAdjBlock), slab ? slab_bytes : 0, st, P, d_xyz,
                        d_gphi, d_gforce, d_gxyz, d_gpoly, d_gscal, W, run_len);
     return hipGetLastError() == hipSuccess ? AGX_OK : AGX_E_LAUNCH;

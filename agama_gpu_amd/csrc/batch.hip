@@ -30,7 +30,7 @@ __global__ void batch_queue_reset(int *q) { if (threadIdx.x == 0) q[0] = 0; }
 
 // rows [start, start + count) with the table at tab_base; each thread reads
 // its next position one iteration ahead
-template <int ORD, bool STRIDE2, bool SIN>
+template <int ORD, int STRIDE, bool SIN>
 __device__ __forceinline__ void batch_rows(const MPK &P, const double *tab_base,
                                            const double *__restrict__ xyz, int64_t start, int count,
                                            double *phi, double *force, double *deriv)
@@ -51,7 +51,7 @@ __device__ __forceinline__ void batch_rows(const MPK &P, const double *tab_base,
         PointOut o{};
         Where w = classify(P, x, y, z);
         if (w.kind == 2)
-            interior<ORD, false, NoHook, STRIDE2, SIN>(P, tab_base + (int64_t)w.k * P.n_lm * 6, x,
+            interior<ORD, false, NoHook, STRIDE, SIN>(P, tab_base + (int64_t)w.k * P.n_lm * 6, x,
                                                        y, z, w.r2, w.r, w.ir, w.lnr, w.k, o, 0.0);
         else
             extrap<ORD>(P, w, x, y, z, o, 0.0);
@@ -59,9 +59,9 @@ __device__ __forceinline__ void batch_rows(const MPK &P, const double *tab_base,
     }
 }
 
-// STRIDE2 / SIN false: every potential's program satisfies AGX_PROG_STRIDE1 /
-// AGX_PROG_COS_ONLY.
-template <int ORD, bool STRIDE2, bool SIN>
+// STRIDE and SIN as for interior(), from the flags every potential's program
+// satisfies.
+template <int ORD, int STRIDE, bool SIN>
 __global__ __launch_bounds__(kBatchBlock) void mp_batch_kernel(
     const char *__restrict__ descs, int64_t stride, int n_pot,
     const agx_batch_item *__restrict__ items, int64_t n_items, const double *__restrict__ xyz,
@@ -89,15 +89,15 @@ __global__ __launch_bounds__(kBatchBlock) void mp_batch_kernel(
             dvec2 *dst = (dvec2 *)tab;
             for (int j = tid; j < tab_len / 2; j += kBatchBlock) dst[j] = src[j];
             __syncthreads();
-            batch_rows<ORD, STRIDE2, SIN>(P, tab, xyz, start, count, phi, force, deriv);
+            batch_rows<ORD, STRIDE, SIN>(P, tab, xyz, start, count, phi, force, deriv);
             __syncthreads();
         } else {
-            batch_rows<ORD, STRIDE2, SIN>(P, P.poly, xyz, start, count, phi, force, deriv);
+            batch_rows<ORD, STRIDE, SIN>(P, P.poly, xyz, start, count, phi, force, deriv);
         }
     }
 }
 
-template <int ORD, bool STRIDE2, bool SIN>
+template <int ORD, int STRIDE, bool SIN>
 static unsigned batch_grid(int64_t n_items)
 {
     static int per_cu[4] = {0, 0, 0, 0};
@@ -112,10 +112,10 @@ static unsigned batch_grid(int64_t n_items)
             n_cu = 256;
     }
     if (!per_cu[ORD]) {
-        hipFuncSetAttribute((const void *)mp_batch_kernel<ORD, STRIDE2, SIN>,
+        hipFuncSetAttribute((const void *)mp_batch_kernel<ORD, STRIDE, SIN>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         int b = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, mp_batch_kernel<ORD, STRIDE2, SIN>,
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, mp_batch_kernel<ORD, STRIDE, SIN>,
                                                          kBatchBlock, lds) != hipSuccess ||
             b < 1)
             b = 1;
@@ -127,13 +127,13 @@ static unsigned batch_grid(int64_t n_items)
     return (unsigned)g;
 }
 
-template <int ORD, bool STRIDE2, bool SIN>
+template <int ORD, int STRIDE, bool SIN>
 static void launch_batch(const char *descs, int64_t stride, int n_pot, const agx_batch_item *items,
                          int64_t n_items, const double *xyz, int64_t n_rows, double *phi,
                          double *force, double *deriv, int *queue, hipStream_t s)
 {
-    hipLaunchKernelGGL((mp_batch_kernel<ORD, STRIDE2, SIN>),
-                       dim3(batch_grid<ORD, STRIDE2, SIN>(n_items)), dim3(kBatchBlock),
+    hipLaunchKernelGGL((mp_batch_kernel<ORD, STRIDE, SIN>),
+                       dim3(batch_grid<ORD, STRIDE, SIN>(n_items)), dim3(kBatchBlock),
                        kBatchTableDoubles * sizeof(double), s, descs, stride, n_pot, items, n_items,
                        xyz, n_rows, phi, force, deriv, queue);
 }
@@ -144,19 +144,18 @@ static void launch_batch(uint32_t flags, const char *descs, int64_t stride, int 
                          int64_t n_rows, double *phi, double *force, double *deriv, int *queue,
                          hipStream_t s)
 {
-    const bool s1 = flags & AGX_PROG_STRIDE1, co = flags & AGX_PROG_COS_ONLY;
-    if (s1 && co)
-        launch_batch<ORD, false, false>(descs, stride, n_pot, items, n_items, xyz, n_rows, phi,
-                                        force, deriv, queue, s);
-    else if (s1)
-        launch_batch<ORD, false, true>(descs, stride, n_pot, items, n_items, xyz, n_rows, phi,
-                                       force, deriv, queue, s);
-    else if (co)
-        launch_batch<ORD, true, false>(descs, stride, n_pot, items, n_items, xyz, n_rows, phi,
-                                       force, deriv, queue, s);
-    else
-        launch_batch<ORD, true, true>(descs, stride, n_pot, items, n_items, xyz, n_rows, phi,
-                                      force, deriv, queue, s);
+    const int st = prog_stride(flags);
+    const bool co = flags & AGX_PROG_COS_ONLY;
+#define AGX_LAUNCH_BATCH(S, C)                                                                  \
+    launch_batch<ORD, S, C>(descs, stride, n_pot, items, n_items, xyz, n_rows, phi, force, deriv, \
+                            queue, s)
+    if (st == 1 && co) AGX_LAUNCH_BATCH(1, false);
+    else if (st == 1) AGX_LAUNCH_BATCH(1, true);
+    else if (st == 2 && co) AGX_LAUNCH_BATCH(2, false);
+    else if (st == 2) AGX_LAUNCH_BATCH(2, true);
+    else if (co) AGX_LAUNCH_BATCH(0, false);
+    else AGX_LAUNCH_BATCH(0, true);
+#undef AGX_LAUNCH_BATCH
 }
 
 }  // namespace agx
@@ -217,7 +216,7 @@ extern "C" int agx_multipole_batch_eval(const void *d_desc, int64_t desc_stride,
     if (n_items < 0 || n_pot < 0 || n_rows < 0) return AGX_E_SIZE;
     if (desc_stride < (int64_t)sizeof(MPK) || (desc_stride & 7)) return AGX_E_SIZE;
     if (what == 0 || (what & ~(AGX_PHI | AGX_FORCE | AGX_DERIV))) return AGX_E_WHAT;
-    if (prog_flags & ~(AGX_PROG_STRIDE1 | AGX_PROG_COS_ONLY)) return AGX_E_WHAT;
+    if (prog_flags & ~kProgFlags) return AGX_E_WHAT;
     if (n_items == 0) return AGX_OK;
     if (!d_desc || !d_items || !d_xyz || !d_queue) return AGX_E_NULL;
     if (((what & AGX_PHI) && !d_phi) || ((what & AGX_FORCE) && !d_force) ||

@@ -53,9 +53,8 @@ __global__ __launch_bounds__(kBinThreads) void mp_binned_kernel(
 // per workgroup: the per-term gathers of the lane kernel become LDS reads.
 // Used for tables of up to 160 KB (the LDS of one CU); the workgroup size is
 // chosen so that the workgroups that fit in LDS hold 16 wavefronts per CU.
-// Without STRIDE2 the program must not recur by 2 (AGX_PROG_STRIDE1); without
-// SIN it must have no sin terms (AGX_PROG_COS_ONLY).
-template <int ORD, int BLOCK, bool STRIDE2, bool SIN>
+// STRIDE and SIN as for interior(): the program flags the variant requires.
+template <int ORD, int BLOCK, int STRIDE, bool SIN>
 __global__ __launch_bounds__(BLOCK) void mp_lds_kernel(MPK P, const double *__restrict__ xyz,
                                                         int64_t n, double *__restrict__ phi,
                                                         double *__restrict__ force,
@@ -83,7 +82,7 @@ __global__ __launch_bounds__(BLOCK) void mp_lds_kernel(MPK P, const double *__re
         PointOut o{};
         Where w = classify(P, x, y, z);
         if (w.kind == 2)
-            interior<ORD, false, NoHook, STRIDE2, SIN>(P, lds_tab + w.k * slab_len, x, y, z,
+            interior<ORD, false, NoHook, STRIDE, SIN>(P, lds_tab + w.k * slab_len, x, y, z,
                                                        w.r2, w.r, w.ir, w.lnr, w.k, o, inv4piG);
         else
             extrap<ORD>(P, w, x, y, z, o, inv4piG);
@@ -129,7 +128,7 @@ static int check_mp(const agx_multipole *mp, const double *xyz, int64_t n)
     if (!mp->poly_km || !mp->program) return AGX_E_NULL;
     if (mp->n_int < 1 || mp->n_lm < 1 || mp->n_groups < 0 || mp->n_steps < 0) return AGX_E_SIZE;
     if (mp->lmax > 32) return AGX_E_LMAX;
-    if (mp->prog_flags & ~(AGX_PROG_STRIDE1 | AGX_PROG_COS_ONLY)) return AGX_E_WHAT;
+    if (mp->prog_flags & ~kProgFlags) return AGX_E_WHAT;
     return AGX_OK;
 }
 
@@ -188,24 +187,24 @@ static unsigned lds_grid(int64_t n, int block, int64_t per_cu)
 }
 
 // Dynamic LDS beyond 64 KB has to be allowed per kernel.
-template <int ORD, int BLOCK, bool STRIDE2, bool SIN>
+template <int ORD, int BLOCK, int STRIDE, bool SIN>
 static void lds_attr(int64_t bytes)
 {
     static int64_t set = 0;
     if (bytes > set) {
-        if (hipFuncSetAttribute((const void *)mp_lds_kernel<ORD, BLOCK, STRIDE2, SIN>,
+        if (hipFuncSetAttribute((const void *)mp_lds_kernel<ORD, BLOCK, STRIDE, SIN>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsPerCu) ==
             hipSuccess)
             set = kLdsPerCu;
     }
 }
 
-template <int ORD, int BLOCK, bool STRIDE2, bool SIN>
+template <int ORD, int BLOCK, int STRIDE, bool SIN>
 static void launch_lds(const MPK &P, unsigned grid, int64_t tb, const double *xyz, int64_t n,
                        double *phi, double *force, double *deriv, double inv4piG, hipStream_t s)
 {
-    if (BLOCK > 256) lds_attr<ORD, BLOCK, STRIDE2, SIN>(tb);
-    hipLaunchKernelGGL((mp_lds_kernel<ORD, BLOCK, STRIDE2, SIN>), dim3(grid), dim3(BLOCK),
+    if (BLOCK > 256) lds_attr<ORD, BLOCK, STRIDE, SIN>(tb);
+    hipLaunchKernelGGL((mp_lds_kernel<ORD, BLOCK, STRIDE, SIN>), dim3(grid), dim3(BLOCK),
                        (size_t)tb, s, P, xyz, n, phi, force, deriv, inv4piG);
 }
 
@@ -214,15 +213,20 @@ static void launch_lds(const agx_multipole *mp, const MPK &P, unsigned grid, int
                        const double *xyz, int64_t n, double *phi, double *force, double *deriv,
                        double inv4piG, hipStream_t s)
 {
-    const bool s1 = mp->prog_flags & AGX_PROG_STRIDE1, co = mp->prog_flags & AGX_PROG_COS_ONLY;
-    if (s1 && co)
-        launch_lds<ORD, BLOCK, false, false>(P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s);
-    else if (s1)
-        launch_lds<ORD, BLOCK, false, true>(P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s);
+    const int st = prog_stride(mp->prog_flags);
+    const bool co = mp->prog_flags & AGX_PROG_COS_ONLY;
+    if (st == 1 && co)
+        launch_lds<ORD, BLOCK, 1, false>(P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s);
+    else if (st == 1)
+        launch_lds<ORD, BLOCK, 1, true>(P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s);
+    else if (st == 2 && co)
+        launch_lds<ORD, BLOCK, 2, false>(P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s);
+    else if (st == 2)
+        launch_lds<ORD, BLOCK, 2, true>(P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s);
     else if (co)
-        launch_lds<ORD, BLOCK, true, false>(P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s);
+        launch_lds<ORD, BLOCK, 0, false>(P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s);
     else
-        launch_lds<ORD, BLOCK, true, true>(P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s);
+        launch_lds<ORD, BLOCK, 0, true>(P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s);
 }
 
 template <int ORD>

@@ -115,7 +115,10 @@ struct NoHook {
 
 // Interior evaluation on radial interval k (slab = poly + k*n_lm*6).
 // UNI: the slab address is wave-uniform (scalar loads of coefficients).
-template <int ORD, bool UNI, class Hook = NoHook, bool STRIDE2 = true, bool SIN = true>
+// STRIDE: 1 or 2 when every group of the program that recurs steps by that
+// stride (AGX_PROG_STRIDE1 / AGX_PROG_STRIDE2), 0 for any program; SIN false
+// when the program has no sin terms (AGX_PROG_COS_ONLY).
+template <int ORD, bool UNI, class Hook = NoHook, int STRIDE = 0, bool SIN = true>
 __device__ __forceinline__ void interior(const MPK &P, const double *slab, double x, double y,
                                          double z, double r2, double r, double ir, double lnr,
                                          int k, PointOut &o, double inv4piG,
@@ -165,6 +168,9 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
     if (R2 >= 0x1p-767) { sqrt_rsqrt(R2, R, iR); cph = x * iR; sph = y * iR; }
     const double ct = z * ir, st = R * ir;
     const bool pole = st < 1.0e-10;
+    // cos^2 theta and its theta derivative, once per point when every
+    // recurring group steps by 2
+    const double ct2 = STRIDE == 2 ? ct * ct : 0.0, dct2 = STRIDE == 2 ? -2.0 * ct * st : 0.0;
     // 1/sin(theta) = r/R, needed inside the term loop only for the Hessian
     const double ist_loop = (ANG2 && !pole) ? r * iR : 0.0;
 
@@ -255,11 +261,14 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
         //   P_l = t P_{l-s} + gamma P_{l-2s},
         //   dP_l/dtheta = t dP_{l-s} + alpha (dc/dtheta) P_{l-s} + gamma dP_{l-2s};
         // beta is 0 where s = 1.
-        // c and dc/dtheta from h = s - 1 by arithmetic rather than a select,
-        // so that no cos^2 or cos sin stays live across the group loop;
-        // without STRIDE2 (no group recurs by 2) c = cos, dc = -sin
+        // for a program of mixed strides, c and dc/dtheta per group from
+        // h = s - 1 by arithmetic rather than a select, so that no cos^2 or
+        // cos sin stays live across the group loop
         double cc = ct, dcc = -st;
-        if (STRIDE2) {
+        if (STRIDE == 2) {
+            cc = ct2;
+            dcc = dct2;
+        } else if (STRIDE == 0) {
             const double h = (double)(gr.s3 - 1);
             const double u = fma(ct, h, 1.0 - h);              // cos or 1
             cc = ct * u;
@@ -267,7 +276,7 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
         }
         auto recur = [&](const ivec8 &rec, double &Qo, double &dQo, double Qn, double dQn) {
             const double al = rec_lo(rec), be = rec_hi(rec), ga = rec_mid(rec);
-            const double t = STRIDE2 ? al * (cc + be) : al * cc;
+            const double t = STRIDE != 1 ? al * (cc + be) : al * cc;
             if (D1) dQo = fma(t, dQn, fma(al * dcc, Qn, ga * dQo));
             Qo = fma(t, Qn, ga * Qo);
         };
@@ -556,6 +565,15 @@ struct MpBinKey {
         return (w.kind == 2) ? w.k + 1 : 0;
     }
 };
+
+constexpr uint32_t kProgFlags = AGX_PROG_STRIDE1 | AGX_PROG_COS_ONLY | AGX_PROG_STRIDE2;
+
+// The stride a kernel variant may assume from the program flags: 1, 2, or 0
+// (any).  A program with no recurring group satisfies both flags.
+inline int prog_stride(uint32_t flags)
+{
+    return (flags & AGX_PROG_STRIDE1) ? 1 : (flags & AGX_PROG_STRIDE2) ? 2 : 0;
+}
 
 inline MPK make_args(const agx_multipole *mp)
 {

@@ -251,13 +251,15 @@ def build_legendre_program(lm_l: np.ndarray, lm_m: np.ndarray) -> LegendreProgra
 
 
 def program_flags(prog: LegendreProgram) -> int:
-    """agx_multipole.prog_flags for this program: AGX_PROG_STRIDE1 (1) when
-    every group that recurs (more than one step) has stride 1, plus
-    AGX_PROG_COS_ONLY (2) when no step has a sin term."""
+    """agx_multipole.prog_flags for this program: AGX_PROG_STRIDE1 (1) /
+    AGX_PROG_STRIDE2 (4) when every group that recurs (more than one step)
+    has stride 1 / 2, plus AGX_PROG_COS_ONLY (2) when no step has a sin term."""
     steps = np.diff(prog.grp_step0)
-    stride1 = not np.any((prog.grp_stride == 2) & (steps > 1))
+    recurs = steps > 1
+    stride1 = not np.any((prog.grp_stride != 1) & recurs)
+    stride2 = not np.any((prog.grp_stride != 2) & recurs)
     cos_only = not np.any(prog.step_sin >= 0)
-    return int(stride1) | (2 if cos_only else 0)
+    return int(stride1) | (2 if cos_only else 0) | (4 if stride2 else 0)
 
 
 # 32 bytes: four int32 (group records: m, step_begin, step_end, stride; step

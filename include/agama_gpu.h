@@ -74,14 +74,17 @@ enum {
  *   prog_flags : an OR of
  *             AGX_PROG_STRIDE1  every group of `program` with more than one
  *                               step has stride 1;
+ *             AGX_PROG_STRIDE2  every group of `program` with more than one
+ *                               step has stride 2;
  *             AGX_PROG_COS_ONLY no step of `program` has a sin term (every
  *                               sin_off is -1: tables with m >= 0 only);
- *             the kernels then run variants without the stride-2 arithmetic
- *             or the sin sums.  0 is always correct; a flag set for a program
- *             that does not satisfy it gives wrong results.
+ *             the kernels then run variants without the per-group stride
+ *             arithmetic or the sin sums.  0 is always correct; a flag set
+ *             for a program that does not satisfy it gives wrong results.
  * ---------------------------------------------------------------------- */
 #define AGX_PROG_STRIDE1 1u
 #define AGX_PROG_COS_ONLY 2u
+#define AGX_PROG_STRIDE2 4u
 
 typedef struct agx_multipole {
     const double *poly_km;

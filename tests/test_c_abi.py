@@ -59,7 +59,9 @@ def test_argument_validation_without_gpu():
     assert L.agx_multipole_eval(C.byref(d), 8, 0, 8, None, None, 1, None) == 0
     d.prog_flags = _lib.AGX_PROG_COS_ONLY
     assert L.agx_multipole_eval(C.byref(d), 8, 0, 8, None, None, 1, None) == 0
-    d.prog_flags = 4                    # no such program flag
+    d.prog_flags = _lib.AGX_PROG_STRIDE2
+    assert L.agx_multipole_eval(C.byref(d), 8, 0, 8, None, None, 1, None) == 0
+    d.prog_flags = 8                    # no such program flag
     assert L.agx_multipole_eval(C.byref(d), 8, 10, 8, None, None, 1, None) == -5
 
 

@@ -117,20 +117,21 @@ def test_empty_and_ragged_sizes(cases, mp_path):
 
 @pytest.mark.parametrize("name", ["fire_dense", "lmc", "halo_lmax8"])
 def test_program_flag_variants_agree(cases, name):
-    """The LDS-staged kernel variants without the stride-2 arithmetic
-    (AGX_PROG_STRIDE1) or without the sin sums (AGX_PROG_COS_ONLY) give the
-    same results as the generic one on the programs the flags describe: for a
-    stride-1 group the generic step's c = cos * fma(cos, 0, 1) and
-    dc = -sin * fma(0, cos, 1) are exact, and without sin terms the generic
-    sums add sin(m phi) x 0.  STRIDE1 is set exactly when no group recurs by
-    2, COS_ONLY exactly when no term has m < 0 (halo_lmax8: every |m| group
-    steps by 2 and holds cos terms only; the dense tables step by 1 and hold
-    both)."""
+    """The LDS-staged kernel variants for programs that recur by 1 only
+    (AGX_PROG_STRIDE1), by 2 only (AGX_PROG_STRIDE2) or have no sin terms
+    (AGX_PROG_COS_ONLY) give the same results as the generic one on the
+    programs the flags describe: the generic step's c = cos * fma(cos, h, 1-h)
+    and dc = -sin * fma(h, cos, u) are exact for h = 0 and h = 1, and without
+    sin terms the generic sums add sin(m phi) x 0.  STRIDE1 / STRIDE2 are set
+    exactly when every recurring group steps by 1 / 2, COS_ONLY exactly when
+    no term has m < 0 (halo_lmax8: every recurring |m| group steps by 2 and
+    holds cos terms only; the dense tables step by 1 and hold both)."""
     from agama_gpu_amd import MultipolePotentialGPU, _lib
     from agama_gpu_amd.multipole_prep import program_flags
     gpu = MultipolePotentialGPU(cases[name])
     prog = gpu._data["program"]
-    want = _lib.AGX_PROG_COS_ONLY if name == "halo_lmax8" else _lib.AGX_PROG_STRIDE1
+    want = (_lib.AGX_PROG_COS_ONLY | _lib.AGX_PROG_STRIDE2) if name == "halo_lmax8" \
+        else _lib.AGX_PROG_STRIDE1
     assert program_flags(prog) == want == gpu._desc.prog_flags
     x = torch.as_tensor(points(100_000, 4), device="cuda")
     _lib.set_option(_lib.AGX_OPT_MP_PATH, 3)
