@@ -204,7 +204,7 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
             cm = t;
             ++cur_m;
         }
-        const double q = rec_lo(gr);
+        const double q = rec_lo(gr), fm = rec_hi(gr);       // qmm and m
         // Two recurrence buffers A and B are used alternately: each step
         // writes the new P_l over the older of the two, so no register copies
         // are needed.  With scalar (UNI) coefficient loads the cos-term
@@ -212,7 +212,7 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
         // SGPRs) while this step computes; per-lane loads are issued at their
         // use, where prefetching costs more VGPRs than it saves in latency.
         double QA = q * sp, dQA = 0.0, QB = 0.0, dQB = 0.0;
-        if (D1 && m > 0) dQA = (double)m * q * sp1 * ct;
+        if (D1) dQA = fm * q * sp1 * ct;                     // 0 for m = 0
         double Vc = 0.0, Vs = 0.0, Rc = 0.0, Rs = 0.0, Tc = 0.0, Ts = 0.0;
         double RRc = 0.0, RRs = 0.0, TTc = 0.0, TTs = 0.0, RTc = 0.0, RTs = 0.0, Lc = 0.0, Ls = 0.0;
         auto add_term = [&](const double a[6], double Q, double dQ, double d2Q, double ll1, double &V,
@@ -317,7 +317,6 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
         if (D1) {
             St = comb(Tc, Ts, St);
             if (m > 0) {
-                const double fm = (double)m;
                 const double Fv = fm * rot(Vs, Vc);                 // dPhi/dphi (group)
                 Sfn += Fv;
                 if (m == 1) Sfp += rot(Ts, Tc);                     // pole limit of Y/sin

@@ -264,7 +264,7 @@ def program_flags(prog: LegendreProgram) -> int:
 
 # 32 bytes: four int32 (group records: m, step_begin, step_end, stride; step
 # records: l, the packed offsets and, over i2-i3, the double gamma), then two
-# doubles (group: qmm, 0; step: alpha, beta).  The step record of l = m is all
+# doubles (group: qmm, m; step: alpha, beta).  The step record of l = m is all
 # coefficients zero.
 PROGRAM_RECORD = np.dtype({"names": ["i0", "i1", "i2", "i3", "g", "d0", "d1"],
                            "formats": ["<i4", "<i4", "<i4", "<i4", "<f8", "<f8", "<f8"],
@@ -299,6 +299,7 @@ def pack_program(prog: LegendreProgram) -> np.ndarray:
     g["i2"] = prog.grp_step0[1:]
     g["i3"] = prog.grp_stride
     g["d0"] = prog.qmm
+    g["d1"] = prog.grp_m
     st = rec[G + 1:G + 1 + S]
     st["i0"] = prog.step_l
     st["i1"] = pack_offsets(np.where(prog.step_cos >= 0, 6 * prog.step_cos, -1),

@@ -55,7 +55,7 @@ enum {
  *             16-byte aligned), n_groups + n_steps + 2 of them:
  *       [0, n_groups)   one per active |m| group, in increasing |m|:
  *                       int32 m, step_begin, step_end, stride s (1 or 2);
- *                       double qmm (mul_m x normalised P_mm / sin^m theta), 0
+ *                       double qmm (mul_m x normalised P_mm / sin^m theta), m
  *       n_groups        all zero
  *       [n_groups + 1, n_groups + 1 + n_steps)   one per recurrence step
  *                       l = m, m + s, .. l_last of each group, in order:

@@ -37,7 +37,7 @@ def test_records_encode_program(lmax, sym):
         # stride 2 exactly when every term of the group has l - m even
         ls = [int(a) for a, b in zip(l[1:], m[1:]) if abs(int(b)) == mm]
         assert stride == (2 if all((a - mm) % 2 == 0 for a in ls) else 1)
-        assert rec[g]["d0"] == prog.qmm[g]
+        assert rec[g]["d0"] == prog.qmm[g] and rec[g]["d1"] == float(mm)
         for j in range(j0, j1):
             s = rec[G + 1 + j]
             assert s["i0"] == mm + stride * (j - j0)
