@@ -144,6 +144,8 @@ __device__ __forceinline__ double agx_log(double x)
 // e^x for |x| < 700, within a few ulp: x = k ln2 + r with |r| <= ln2/2
 // (Cody-Waite reduction, ln 2 in two parts), e^r by its Taylor series to r^13
 // (the next term is below 2^-57), scaled by 2^k.  About 20 VALU instructions.
+// Beyond that range (|x| up to about 1e9) the scaling by 2^k underflows to 0
+// or overflows to infinity as e^x does.
 __device__ __forceinline__ double agx_exp(double x)
 {
     const double k = __builtin_rint(x * sconst(1.44269504088896338700e+00));

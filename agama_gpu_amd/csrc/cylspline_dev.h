@@ -288,7 +288,7 @@ __device__ __forceinline__ void cylspline_point(const CSK &P, double x, double y
     // ---- physical derivatives in (lR, lz, phi) ------------------------------
     double Pv, PR, PZ, Pp, PRR = 0, PZZ = 0, PRZ = 0, PRp = 0, PZp = 0, Ppp = 0;
     if (P.log_scaling) {
-        const double P0 = -exp(F0);
+        const double P0 = -agx_exp(F0);
         const double P0R = P0 * F0R, P0Z = P0 * F0Z;
         Pv = P0 * S;
         PR = fma(P0, SR, P0R * S);

@@ -63,7 +63,7 @@ __device__ __forceinline__ double epow(double a, double t, double er)
     if (a == 0.0) return 1.0;
     if (a == -1.0) return rcp_nr(er);
     if (a == 2.0) return er * er;
-    return exp(a * t);
+    return agx_exp(a * t);
 }
 
 __device__ __forceinline__ void law(double v, double s, double U, double W, int logm, double t,
