@@ -37,6 +37,7 @@ struct MPK {
     double logr_min, dlogr, inv_dlogr, invPhi0;
     double inner_s, inner_U, inner_W, outer_s, outer_U, outer_W;
     double inv_rmin, inv_rmax;       // 1 / r at the first and last node
+    double rmax;                     // r at the last node
     const double *law;               // NULL or device [invPhi0, inner U, W, outer U, W]
 };
 
@@ -56,27 +57,28 @@ __device__ __forceinline__ MPK with_law(MPK P)
 // l=0 extrapolation law and its first two ln r derivatives.
 //   log mode:   Phi = (W + U t) e^{v t}
 //   power mode: Phi = W e^{v t} + U e^{s t}
-// t = ln(r / r_edge) and er = r / r_edge = e^t: the exponents that occur in
-// practice (0, -1, 2) are powers of er, which avoids the exponential.
-__device__ __forceinline__ double epow(double a, double t, double er)
+// t = ln(r / r_edge), er = r / r_edge = e^t and ier = 1 / er: the exponents
+// that occur in practice (0, -1, 2) are powers of er, which avoids the
+// exponential.
+__device__ __forceinline__ double epow(double a, double t, double er, double ier)
 {
     if (a == 0.0) return 1.0;
-    if (a == -1.0) return rcp_nr(er);
+    if (a == -1.0) return ier;
     if (a == 2.0) return er * er;
     return agx_exp(a * t);
 }
 
 __device__ __forceinline__ void law(double v, double s, double U, double W, int logm, double t,
-                                    double er, double &P, double &D, double &D2)
+                                    double er, double ier, double &P, double &D, double &D2)
 {
-    double ev = epow(v, t, er);
+    double ev = epow(v, t, er, ier);
     if (logm) {
         double q = fma(U, t, W);
         P = q * ev;
         D = fma(v, q, U) * ev;
         D2 = fma(v * v, q, 2.0 * v * U) * ev;
     } else {
-        double es = epow(s, t, er);
+        double es = epow(s, t, er, ier);
         double wv = W * ev, us = U * es;
         P = wv + us;
         D = fma(v, wv, s * us);
@@ -85,14 +87,14 @@ __device__ __forceinline__ void law(double v, double s, double U, double W, int 
 }
 
 // Radial/extrapolated part: fills `o` for points with a purely radial l=0
-// potential (inside r_min or beyond r_max).  dP, d2P are d/d ln r.
+// potential (inside r_min or beyond r_max).  dP, d2P are d/d ln r; ir = 1/r.
 template <int ORD>
-__device__ __forceinline__ void radial_out(double x, double y, double z, double r2,
+__device__ __forceinline__ void radial_out(double x, double y, double z, double ir,
                                            double P, double D, double D2, PointOut &o)
 {
     o.phi = P;
     if (ORD == ORD_GRAD || ORD == ORD_HESS) {
-        double ir2 = 1.0 / r2, g = D * ir2;
+        double ir2 = ir * ir, g = D * ir2;
         o.g[0] = g * x; o.g[1] = g * y; o.g[2] = g * z;
         if (ORD == ORD_HESS) {
             // H_ij = (D2 - 2D) x_i x_j / r^4 + D delta_ij / r^2 for a radial Phi
@@ -404,9 +406,9 @@ __device__ __forceinline__ Where classify(const MPK &P, double x, double y, doub
     w.r2 = fma(x, x, fma(y, y, z * z));
     if (w.r2 >= 0x1p-767) {
         sqrt_rsqrt(w.r2, w.r, w.ir);
-    } else {                  // r < 4e-116: the origin or the inner law
+    } else {                  // r < 4e-116: the origin (ir unused) or the inner law
         w.r = sqrt(w.r2);
-        w.ir = 0.0;
+        w.ir = 1.0 / w.r;
     }
     w.k = 0;
     w.lnr = 0.0;
@@ -435,15 +437,17 @@ __device__ __forceinline__ void extrap(const MPK &P, const Where &w, double x, d
         return;
     }
     if (w.kind == 1) {
-        law(0.0, P.inner_s, P.inner_U, P.inner_W, P.inner_log, w.lnr - P.logr_min, w.r * P.inv_rmin,
-            Pv, D, D2);
+        const double er = w.r * P.inv_rmin;
+        law(0.0, P.inner_s, P.inner_U, P.inner_W, P.inner_log, w.lnr - P.logr_min, er,
+            rcp_nr(er), Pv, D, D2);
     } else {
         if (ORD == ORD_DENS) { o.phi = 0.0; return; }
         law(-1.0, P.outer_s, P.outer_U, P.outer_W, P.outer_log,
-            w.lnr - fma((double)P.n_int, P.dlogr, P.logr_min), w.r * P.inv_rmax, Pv, D, D2);
+            w.lnr - fma((double)P.n_int, P.dlogr, P.logr_min), w.r * P.inv_rmax, P.rmax * w.ir,
+            Pv, D, D2);
     }
-    if (ORD == ORD_DENS) { o.phi = inv4piG * (D2 + D) / w.r2; return; }
-    radial_out<ORD>(x, y, z, w.r2, Pv, D, D2, o);
+    if (ORD == ORD_DENS) { o.phi = inv4piG * (D2 + D) * (w.ir * w.ir); return; }
+    radial_out<ORD>(x, y, z, w.ir, Pv, D, D2, o);
 }
 
 template <int ORD>
@@ -587,6 +591,7 @@ inline MPK make_args(const agx_multipole *mp)
     P.outer_s = mp->outer_s; P.outer_U = mp->outer_U; P.outer_W = mp->outer_W;
     P.inv_rmin = std::exp(-mp->logr_min);
     P.inv_rmax = std::exp(-(mp->logr_min + (double)mp->n_int * mp->dlogr));
+    P.rmax = std::exp(mp->logr_min + (double)mp->n_int * mp->dlogr);
     P.law = mp->law;
     return P;
 }
