@@ -317,17 +317,18 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
         Sv = comb(Vc, Vs, Sv);
         if (R1) Sr = comb(Rc, Rs, Sr);
         if (D1) {
+            // every phi-derivative term carries the factor m (0 for m = 0,
+            // where also sin(m phi) = 0), so no test of m is needed
             St = comb(Tc, Ts, St);
-            if (m > 0) {
-                const double Fv = fm * rot(Vs, Vc);                 // dPhi/dphi (group)
-                Sfn += Fv;
-                if (m == 1) Sfp += rot(Ts, Tc);                     // pole limit of Y/sin
-                if (ANG2) {
-                    Sf += Fv;
-                    Sff -= fm * fm * (SIN ? fma(cm, Vc, sm * Vs) : cm * Vc);
-                    Srf += fm * rot(Rs, Rc);
-                    Stf += fm * rot(Ts, Tc);
-                }
+            const double Fv = fm * rot(Vs, Vc);                     // dPhi/dphi (group)
+            Sfn += Fv;
+            // pole limit of Y/sin: the m = 1 group only (a scalar 0/1 weight)
+            Sfp = fma(m == 1 ? 1.0 : 0.0, rot(Ts, Tc), Sfp);
+            if (ANG2) {
+                Sf += Fv;
+                Sff -= fm * fm * (SIN ? fma(cm, Vc, sm * Vs) : cm * Vc);
+                Srf += fm * rot(Rs, Rc);
+                Stf += fm * rot(Ts, Tc);
             }
         }
         if (D2) Srr = comb(RRc, RRs, Srr);
