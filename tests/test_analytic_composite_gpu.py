@@ -137,3 +137,32 @@ def test_factory_ini(tmp_path):
     ref = A.nfw(x, 1e12, 20.0)[0] + A.miyamoto_nagai(x, 6e10, 3.0, 0.3)[0] + \
         lmc.potential(torch.as_tensor(x, device="cuda")).cpu().numpy()
     assert np.max(np.abs(p - ref) / np.abs(ref)) < 1e-12
+
+
+@pytest.mark.parametrize("n", [1, 255, 257, 2_500_001])
+def test_analytic_tiles(n):
+    """Analytic-only sums run a kernel that moves 256-particle tiles through
+    LDS with a grid of at most 8192 workgroups: sizes below one tile, one
+    past a tile, and beyond 8192 tiles (some workgroups take two, the last
+    one partial) for every output (phi, force, deriv, density), with two
+    leaves (NFW + shifted Plummer), against the oracle."""
+    from agama_gpu_amd import PotentialGPU
+    print(f"analytic tiles n={n}", flush=True)
+    from agama_gpu_amd.workloads import hernquist_points
+    x = hernquist_points(max(n, 8), 7)[:n]
+    nfw = dict(mass=1e12, scaleRadius=20.0)
+    plu = dict(mass=1e10, scaleRadius=0.5)
+    pot = PotentialGPU(type="NFW", **nfw) + PotentialGPU(type="Plummer", center=[1.0, -2.0, 0.5],
+                                                        **plu)
+    c = np.array([1.0, -2.0, 0.5])
+    r1, r2 = A.evaluate("NFW", x, **nfw), A.evaluate("Plummer", x - c, **plu)
+    ref = [r1[k] + r2[k] for k in range(4)]
+    xt = torch.as_tensor(x, device="cuda")
+    cmp(pot.evalDeriv(xt), ref, f"evalDeriv n={n}")
+    phi, F = pot.eval(xt, pot=True, acc=True)
+    assert np.max(np.abs(phi.cpu().numpy() - ref[0]) / np.abs(ref[0])) < TOL_P
+    assert np.max(np.abs(F.cpu().numpy() + ref[1])) <= TOL_G * np.max(np.abs(ref[1]))
+    p = pot.potential(xt).cpu().numpy()
+    assert np.max(np.abs(p - ref[0]) / np.abs(ref[0])) < TOL_P
+    rho = pot.density(xt).cpu().numpy()
+    assert np.max(np.abs(rho - ref[3])) <= TOL_G * np.max(np.abs(ref[3]))
