@@ -5,7 +5,9 @@ namespace agx {
 int g_grid_per_cu = 32;
 }
 
-extern "C" int agx_version(void) { return 10000; }  // 1.0.0
+// 1.1.0: agx_multipole.prog_flags, the program records' packed offsets and
+// gamma, the prog_flags argument of agx_multipole_batch_eval
+extern "C" int agx_version(void) { return 10100; }
 
 extern "C" const char *agx_error_string(int code)
 {

@@ -396,7 +396,8 @@ int agx_leapfrog(const agx_component *d_leaves, int32_t n_leaves,
                                  per 4096 points, 64 to 1024) */
 int agx_set_option(int key, int64_t value);
 
-/* Library identification and error text. */
+/* Library identification and error text.  agx_version: major * 10000 +
+ * minor * 100 + patch (10100: this header). */
 int agx_version(void);
 const char *agx_error_string(int code);
 
