@@ -145,3 +145,36 @@ def test_program_flag_variants_agree(cases, name):
         _lib.set_option(_lib.AGX_OPT_MP_PATH, 0)
     for a, b in zip(*outs):
         assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("cos_only", [False, True])
+def test_mixed_stride_program(cases, cos_only, mp_path):
+    """A program whose |m| groups step by different strides: the dense table
+    with its odd-l m = 0 harmonics zeroed (pruned), so the m = 0 group steps
+    by 2 while the others step by 1; with cos_only also the sin harmonics
+    zeroed.  No stride flag applies, so every path runs the per-group stride
+    arithmetic; against the oracle on the same zeroed table."""
+    from agama_gpu_amd import MultipolePotentialGPU, _lib
+    from agama_gpu_amd.multipole_prep import program_flags
+    from oracle.oracle import OracleMultipole
+
+    def keep(l, m):
+        return not (m == 0 and l % 2 == 1) and (m >= 0 or not cos_only)
+    c = cases["fire_dense"].zeroed(keep)
+    print(f"mixed strides cos_only={cos_only} path={mp_path}", flush=True)
+    gpu = MultipolePotentialGPU(c)
+    prog = gpu._data["program"]
+    assert set(prog.grp_stride[np.diff(prog.grp_step0) > 1].tolist()) == {1, 2}
+    assert program_flags(prog) == (_lib.AGX_PROG_COS_ONLY if cos_only else 0)
+    orc = OracleMultipole.from_coefs(c)
+    x = points(50_000, 3)
+    xt = torch.as_tensor(x, device="cuda")
+    p, F, D = [t.cpu().numpy() for t in gpu.evalDeriv(xt)]
+    po, Fo, Do = orc.eval_deriv(x)
+    assert rel_phi(p, po) < TOL_PHI
+    assert rel_glob(F, Fo) < TOL_F
+    R = np.hypot(x[:, 0], x[:, 1])
+    ok = (R == 0) | (R > 1e-6 * np.linalg.norm(x, axis=1))
+    assert rel_glob(D[ok], Do[ok]) < TOL_D
+    p2, F2 = [t.cpu().numpy() for t in gpu.eval(xt, pot=True, acc=True)]
+    assert rel_phi(p2, po) < TOL_PHI and rel_glob(F2, Fo) < TOL_F
