@@ -1,7 +1,7 @@
 /*
  * ORACLE -- test infrastructure only.  Nothing in the product links, loads or
  * calls this file; only tests/, __graft_entry__.smoke() and bench.py's
- * cpu_baseline leg do, as the checker.
+ * cpu_baseline leg do, to check the GPU results and to time a CPU baseline.
  *
  * Plain-C, FP64, OpenMP restatement of the reference's Multipole path:
  *   table build   gpu_potential.py:183-626   (_compute_invPhi0,
