@@ -40,10 +40,11 @@ __global__ __launch_bounds__(kBlock) void composite_kernel(CompositeArgs A,
             const agx_component &C = A.c[c];
             add_leaf<ORD>(C.type, C.p, C.table, C.ampl, C.scale, C.center, x0, y0, z0, P, g, h);
         }
-        if (phi) phi[i] = P;
-        if (G && force) { force[3 * i] = -g[0]; force[3 * i + 1] = -g[1]; force[3 * i + 2] = -g[2]; }
+        if (phi) __builtin_nontemporal_store(P, phi + i);        // streaming: written once
+        if (G && force)
+            for (int k = 0; k < 3; ++k) __builtin_nontemporal_store(-g[k], force + 3 * i + k);
         if (H && deriv)
-            for (int k = 0; k < 6; ++k) deriv[6 * i + k] = -h[k];
+            for (int k = 0; k < 6; ++k) __builtin_nontemporal_store(-h[k], deriv + 6 * i + k);
     });
 }
 
@@ -80,7 +81,7 @@ __global__ __launch_bounds__(kBlock) void analytic_kernel(CompositeArgs A,
     auto flush = [&](double *out, int m, int64_t b) {
         const int64_t avail = (int64_t)m * (n - b);
         for (int j = t; j < m * kBlock; j += kBlock)
-            if (j < avail) out[(int64_t)m * b + j] = L[j];
+            if (j < avail) __builtin_nontemporal_store(L[j], out + (int64_t)m * b + j);
     };
     if (base < n) fetch(base);
     for (; base < n; base += tstride) {
@@ -97,7 +98,7 @@ __global__ __launch_bounds__(kBlock) void analytic_kernel(CompositeArgs A,
                 add_leaf<ORD, false>(C.type, C.p, C.table, C.ampl, C.scale, C.center, x0, y0, z0,
                                      P, g, h);
             }
-            if (phi) phi[i] = P;
+            if (phi) __builtin_nontemporal_store(P, phi + i);
         }
         __syncthreads();                   // the positions in L are read
         if (G && force) {

@@ -12,12 +12,12 @@ __global__ __launch_bounds__(kBlock) void cyl_kernel(CSK P, const double *__rest
     for_each_point<kBlock>(xyz, n, [&](int64_t i, double x, double y, double z) {
         PointOut o{};
         cylspline_point<ORD>(P, x, y, z, o);
-        if (phi) phi[i] = o.phi;
-        if ((ORD == ORD_GRAD || ORD == ORD_HESS) && force) {
-            force[3 * i] = -o.g[0]; force[3 * i + 1] = -o.g[1]; force[3 * i + 2] = -o.g[2];
-        }
+        // streaming (non-temporal) stores: written once, not read again
+        if (phi) __builtin_nontemporal_store(o.phi, phi + i);
+        if ((ORD == ORD_GRAD || ORD == ORD_HESS) && force)
+            for (int k = 0; k < 3; ++k) __builtin_nontemporal_store(-o.g[k], force + 3 * i + k);
         if (ORD == ORD_HESS && deriv)
-            for (int k = 0; k < 6; ++k) deriv[6 * i + k] = -o.h[k];
+            for (int k = 0; k < 6; ++k) __builtin_nontemporal_store(-o.h[k], deriv + 6 * i + k);
     });
 }
 
