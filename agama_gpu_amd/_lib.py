@@ -61,7 +61,7 @@ class agx_cylspline(C.Structure):
 
 class agx_component(C.Structure):
     _fields_ = [
-        ("type", C.c_int32), ("pad", C.c_int32), ("p", C.c_double * 8),
+        ("type", C.c_int32), ("prog_flags", C.c_uint32), ("p", C.c_double * 8),
         ("table", C.c_void_p), ("ampl", C.c_double), ("scale", C.c_double),
         ("center", C.c_double * 3),
     ]

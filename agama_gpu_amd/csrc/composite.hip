@@ -25,7 +25,9 @@ struct CompositeArgs {
     agx_component c[kMaxComponents];
 };
 
-template <int ORD>
+// STRIDE and SIN as for interior(), from the program flags every Multipole
+// leaf's table satisfies.
+template <int ORD, int STRIDE, bool SIN>
 __global__ __launch_bounds__(kBlock) void composite_kernel(CompositeArgs A,
                                                            const double *__restrict__ xyz,
                                                            int64_t n, double *__restrict__ phi,
@@ -38,7 +40,8 @@ __global__ __launch_bounds__(kBlock) void composite_kernel(CompositeArgs A,
         double P = 0.0, g[3] = {0.0, 0.0, 0.0}, h[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         for (int c = 0; c < A.n; ++c) {
             const agx_component &C = A.c[c];
-            add_leaf<ORD>(C.type, C.p, C.table, C.ampl, C.scale, C.center, x0, y0, z0, P, g, h);
+            add_leaf<ORD, true, STRIDE, SIN>(C.type, C.p, C.table, C.ampl, C.scale, C.center, x0,
+                                             y0, z0, P, g, h);
         }
         if (phi) __builtin_nontemporal_store(P, phi + i);        // streaming: written once
         if (G && force)
@@ -119,12 +122,13 @@ __global__ __launch_bounds__(kBlock) void analytic_kernel(CompositeArgs A,
 }
 
 static int check_components(const agx_component *comps, int n_comps, CompositeArgs &A,
-                            bool &any_expansion)
+                            bool &any_expansion, uint32_t &mp_flags)
 {
     if (n_comps < 1 || n_comps > kMaxComponents) return AGX_E_SIZE;
     if (!comps) return AGX_E_NULL;
     A.n = n_comps;
     any_expansion = false;
+    mp_flags = kProgFlags;                 // AND over the Multipole leaves
     for (int c = 0; c < n_comps; ++c) {
         const agx_component &C = comps[c];
         const bool expansion = C.type == AGX_T_MULTIPOLE || C.type == AGX_T_CYLSPLINE;
@@ -132,18 +136,33 @@ static int check_components(const agx_component *comps, int n_comps, CompositeAr
         if (!expansion && !analytic) return AGX_E_TYPE;
         if (expansion && !C.table) return AGX_E_NULL;
         any_expansion = any_expansion || expansion;
+        if (C.type == AGX_T_MULTIPOLE) {
+            if (C.prog_flags & ~kProgFlags) return AGX_E_WHAT;
+            mp_flags &= C.prog_flags;
+        }
         A.c[c] = C;
     }
     return AGX_OK;
 }
 
 template <int ORD>
-static void launch_composite(const CompositeArgs &A, bool expansion, const double *xyz, int64_t n,
-                             double *phi, double *force, double *deriv, hipStream_t s)
+static void launch_composite(const CompositeArgs &A, bool expansion, uint32_t mp_flags,
+                             const double *xyz, int64_t n, double *phi, double *force,
+                             double *deriv, hipStream_t s)
 {
-    if (expansion)
-        hipLaunchKernelGGL(composite_kernel<ORD>, dim3(grid_for(n)), dim3(kBlock), 0, s, A, xyz,
-                           n, phi, force, deriv);
+    // three Multipole variants: dense programs (stride 1), the m >= 0 same-parity
+    // programs of triaxial and axisymmetric tables (stride 2, no sin terms), and
+    // the generic code for anything else
+    const int st = prog_stride(mp_flags);
+    if (expansion && st == 1)
+        hipLaunchKernelGGL((composite_kernel<ORD, 1, true>), dim3(grid_for(n)), dim3(kBlock), 0,
+                           s, A, xyz, n, phi, force, deriv);
+    else if (expansion && st == 2 && (mp_flags & AGX_PROG_COS_ONLY))
+        hipLaunchKernelGGL((composite_kernel<ORD, 2, false>), dim3(grid_for(n)), dim3(kBlock), 0,
+                           s, A, xyz, n, phi, force, deriv);
+    else if (expansion)
+        hipLaunchKernelGGL((composite_kernel<ORD, 0, true>), dim3(grid_for(n)), dim3(kBlock), 0,
+                           s, A, xyz, n, phi, force, deriv);
     else
         hipLaunchKernelGGL(analytic_kernel<ORD>, dim3(grid_for(n)), dim3(kBlock), 0, s, A, xyz, n,
                            phi, force, deriv);
@@ -159,7 +178,8 @@ extern "C" int agx_composite_eval(const agx_component *comps, int n_comps, const
 {
     CompositeArgs A;
     bool exp_leaves;
-    int rc = check_components(comps, n_comps, A, exp_leaves);
+    uint32_t mp_flags;
+    int rc = check_components(comps, n_comps, A, exp_leaves, mp_flags);
     if (rc) return rc;
     if (n < 0) return AGX_E_SIZE;
     if (what == 0 || (what & ~(AGX_PHI | AGX_FORCE | AGX_DERIV))) return AGX_E_WHAT;
@@ -172,11 +192,11 @@ extern "C" int agx_composite_eval(const agx_component *comps, int n_comps, const
     double *deriv = (what & AGX_DERIV) ? d_deriv : nullptr;
     hipStream_t s = (hipStream_t)stream;
     if (deriv)
-        launch_composite<ORD_HESS>(A, exp_leaves, d_xyz, n, phi, force, deriv, s);
+        launch_composite<ORD_HESS>(A, exp_leaves, mp_flags, d_xyz, n, phi, force, deriv, s);
     else if (force)
-        launch_composite<ORD_GRAD>(A, exp_leaves, d_xyz, n, phi, force, nullptr, s);
+        launch_composite<ORD_GRAD>(A, exp_leaves, mp_flags, d_xyz, n, phi, force, nullptr, s);
     else
-        launch_composite<ORD_POT>(A, exp_leaves, d_xyz, n, phi, nullptr, nullptr, s);
+        launch_composite<ORD_POT>(A, exp_leaves, mp_flags, d_xyz, n, phi, nullptr, nullptr, s);
     return hipGetLastError() == hipSuccess ? AGX_OK : AGX_E_LAUNCH;
 }
 
@@ -185,12 +205,13 @@ extern "C" int agx_composite_density(const agx_component *comps, int n_comps, co
 {
     CompositeArgs A;
     bool exp_leaves;
-    int rc = check_components(comps, n_comps, A, exp_leaves);
+    uint32_t mp_flags;
+    int rc = check_components(comps, n_comps, A, exp_leaves, mp_flags);
     if (rc) return rc;
     if (n < 0) return AGX_E_SIZE;
     if (n == 0) return AGX_OK;
     if (!d_xyz || !d_rho) return AGX_E_NULL;
-    launch_composite<ORD_DENS>(A, exp_leaves, d_xyz, n, d_rho, nullptr, nullptr,
+    launch_composite<ORD_DENS>(A, exp_leaves, mp_flags, d_xyz, n, d_rho, nullptr, nullptr,
                                (hipStream_t)stream);
     return hipGetLastError() == hipSuccess ? AGX_OK : AGX_E_LAUNCH;
 }

@@ -13,7 +13,7 @@ namespace agx {
 
 // EXP = false: the caller guarantees an analytic type (no expansion code is
 // compiled in, which keeps analytic-only kernels small in registers).
-template <int ORD, bool EXP = true>
+template <int ORD, bool EXP = true, int STRIDE = 0, bool SIN = true>
 __device__ __forceinline__ void add_leaf(int type, const double *p, const void *table, double ampl,
                                          double sg, const double *center, double x0, double y0,
                                          double z0, double &P, double g[3], double h[6])
@@ -28,8 +28,9 @@ __device__ __forceinline__ void add_leaf(int type, const double *p, const void *
         PointOut o{};
         Where w = classify(mp, x, y, z);
         if (w.kind == 2)
-            interior<ORD, false>(mp, mp.poly + (int64_t)w.k * mp.n_lm * 6, x, y, z, w.r2, w.r,
-                                 w.ir, w.lnr, w.k, o, 1.0 / (4.0 * M_PI * G_AGAMA_D));
+            interior<ORD, false, NoHook, STRIDE, SIN>(mp, mp.poly + (int64_t)w.k * mp.n_lm * 6, x,
+                                                      y, z, w.r2, w.r, w.ir, w.lnr, w.k, o,
+                                                      1.0 / (4.0 * M_PI * G_AGAMA_D));
         else
             extrap<ORD>(mp, w, x, y, z, o, 1.0 / (4.0 * M_PI * G_AGAMA_D));
         if (RHO) e.rho = o.phi;

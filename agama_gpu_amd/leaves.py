@@ -32,6 +32,7 @@ class Leaf:
     ampl: float = 1.0
     sigma: float = 1.0
     tau: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    prog_flags: int = 0                  # Multipole: agx_multipole.prog_flags of the table
 
     def shifted(self, c) -> "Leaf":
         return replace(self, tau=(self.tau[0] + c[0], self.tau[1] + c[1], self.tau[2] + c[2]))
@@ -49,6 +50,7 @@ def to_components(leaves: Sequence[Leaf]):
     for i, lf in enumerate(leaves):
         c = arr[i]
         c.type = lf.type
+        c.prog_flags = lf.prog_flags
         for k, v in enumerate(lf.params):
             c.p[k] = float(v)
         c.table = lf.table or None

@@ -113,7 +113,8 @@ class MultipolePotentialGPU(PotentialBase):
     # ---- primitives ---------------------------------------------------------
     def _leaves(self, t=0.0):
         from .leaves import Leaf
-        return [Leaf(_lib.AGX_T_MULTIPOLE, table=self._dev_desc.data_ptr(), keepalive=self)]
+        return [Leaf(_lib.AGX_T_MULTIPOLE, table=self._dev_desc.data_ptr(), keepalive=self,
+                     prog_flags=int(self._desc.prog_flags))]
 
     def _evaluate(self, xyz, want_phi, want_force, want_deriv, t=0.0, out=None):
         n = xyz.shape[0]

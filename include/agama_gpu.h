@@ -192,7 +192,8 @@ enum {
 
 typedef struct agx_component {
     int32_t type;
-    int32_t pad;
+    uint32_t prog_flags;     /* AGX_T_MULTIPOLE: the table's agx_multipole.prog_flags
+                                (0 is always correct); 0 for the other types */
     double p[8];
     const void *table;
     double ampl;

@@ -65,6 +65,21 @@ def test_argument_validation_without_gpu():
     assert L.agx_multipole_eval(C.byref(d), 8, 10, 8, None, None, 1, None) == -5
 
 
+def test_component_flags_validation_without_gpu():
+    """A Multipole component with an unknown program flag bit is rejected on
+    the host before any launch."""
+    from agama_gpu_amd import _lib
+    L = _lib.load()
+    comps = (_lib.agx_component * 1)()
+    comps[0].type = _lib.AGX_T_MULTIPOLE
+    comps[0].table = 8
+    comps[0].ampl = comps[0].scale = 1.0
+    comps[0].prog_flags = _lib.AGX_PROG_COS_ONLY | _lib.AGX_PROG_STRIDE2
+    assert L.agx_composite_eval(comps, 1, 8, 0, 8, None, None, 1, None) == 0
+    comps[0].prog_flags = 8                # no such program flag
+    assert L.agx_composite_eval(comps, 1, 8, 10, 8, None, None, 1, None) == -5
+
+
 def test_descriptor_layout_matches_header():
     """The ctypes mirror of agx_multipole has the header's field order and
     C layout (no implicit padding before the doubles)."""

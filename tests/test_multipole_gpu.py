@@ -147,6 +147,37 @@ def test_program_flag_variants_agree(cases, name):
         assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("name", ["fire_dense", "lmc", "halo_lmax8"])
+def test_composite_program_flag_variants(cases, name):
+    """The fused composite kernel picks its Multipole code from the AND of the
+    leaves' program flags: with a table's own flags it gives bit-for-bit the
+    result of the generic code (flags 0), and a composite of a stride-1 and a
+    stride-2 table (AND = 0) equals the sum of the two tables' own results.
+    The exact origin is left out: the NFW leaf's force is NaN at r = 0."""
+    from agama_gpu_amd import MultipolePotentialGPU, PotentialGPU
+    gpu = MultipolePotentialGPU(cases[name])
+    nfw = PotentialGPU(type="NFW", mass=1e12, scaleRadius=20.0)
+    x = torch.as_tensor(points(50_000, 7)[1:], device="cuda")
+    own = gpu._desc.prog_flags
+    assert own != 0
+    outs = []
+    try:
+        for flags in (own, 0):
+            gpu._desc.prog_flags = flags
+            outs.append((gpu + nfw).evalDeriv(x))
+    finally:
+        gpu._desc.prog_flags = own
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)
+    other = MultipolePotentialGPU(cases["fire_dense" if name == "halo_lmax8" else "halo_lmax8"])
+    assert gpu._desc.prog_flags & other._desc.prog_flags == 0
+    mixed = (gpu + other).evalDeriv(x)
+    parts = [p.evalDeriv(x) for p in (gpu, other)]
+    for k, m in enumerate(mixed):
+        ref = parts[0][k] + parts[1][k]
+        assert torch.max(torch.abs(m - ref)) <= 1e-13 * torch.max(torch.abs(ref))
+
+
 @pytest.mark.parametrize("cos_only", [False, True])
 def test_mixed_stride_program(cases, cos_only, mp_path):
     """A program whose |m| groups step by different strides: the dense table
