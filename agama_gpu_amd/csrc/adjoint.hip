@@ -54,12 +54,13 @@ struct AdjHook {
     double *gslab;
     double *sum;                 // [4]: l > 0 sums of value, d/ds, d/dtheta, d/dphi
     double s, lam, mu, idl, wphi, wPr, wPt, wPf;
+    double psg;                  // sign of cos(theta): the pole limit's sign
     int lane;
     __device__ void side(int off, double Q, double dQ, int m, double T, double dT, double dTp,
                          double ist, bool pole) const
     {
         if (off < 0) return;
-        const double X = pole ? ((m == 1) ? dQ * dTp : 0.0) : Q * dT * ist;
+        const double X = pole ? ((m == 1) ? psg * dQ * dTp : 0.0) : Q * dT * ist;
         const double QT = Q * T;
         const double alpha = lam * (wphi * QT + wPt * dQ * T + wPf * X) + wPr * mu * QT;
         const double beta = wPr * lam * idl * QT;
@@ -126,7 +127,8 @@ __device__ __forceinline__ void adjoint_interior(const MPK &P, int k, bool valid
 
     // evaluation with the Hessian (for dL/dx) and the per-term scatter
     double sum[5];
-    const AdjHook<SLAB> hook{gslab, sum, s, lam, mu, idl, wphi, wPr, wPt, wPf, lane};
+    const double psg = ct < 0.0 ? -1.0 : 1.0;
+    const AdjHook<SLAB> hook{gslab, sum, s, lam, mu, idl, wphi, wPr, wPt, wPf, psg, lane};
     {
         PointOut o{};
         interior<ORD_HESS, true, AdjHook<SLAB>, STRIDE>(P, slab, x, y, z, r2, r, ir, lnr, k, o,
@@ -138,7 +140,7 @@ __device__ __forceinline__ void adjoint_interior(const MPK &P, int k, bool valid
         }
     }
     const double Sv = sum[0], Sr = sum[1], St = sum[2];
-    const double Sx = pole ? sum[4] : sum[3] * ist;
+    const double Sx = pole ? psg * sum[4] : sum[3] * ist;
     // monopole coefficients and invPhi0
     double gc0, gc0d, ginv = 0.0;
     if (P.log_scaling) {
@@ -166,6 +168,8 @@ __device__ __forceinline__ void adjoint_extrap(const MPK &P, bool valid, double 
         Where w = classify(P, x, y, z);
         if (w.kind == 0) {
             gW_in = wphi;
+            gx[0] = gx[1] = gx[2] = 0.0;
+        } else if (beyond_fp64(w)) {     // r = inf: Phi and F are 0 for every table
             gx[0] = gx[1] = gx[2] = 0.0;
         } else {
             const bool in = w.kind == 1;
@@ -267,8 +271,8 @@ extern "C" int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xy
     if (mp->lmax > 32) return AGX_E_LMAX;
     if (mp->prog_flags & ~kProgFlags) return AGX_E_WHAT;
     if (n == 0) return AGX_OK;
-    if (!d_xyz || (!d_gphi && !d_gforce) || !d_work) return AGX_E_NULL;
     if (work_bytes < sort_work_size(n, kMaxBins)) return AGX_E_WORKSPACE;
+    if (!d_xyz || (!d_gphi && !d_gforce) || !d_work) return AGX_E_NULL;
     MPK P = make_args(mp);
     const SortWork W = sort_work(d_work, n, kMaxBins);
     const int nb = mp->n_int + 1;

@@ -27,33 +27,61 @@ struct Eval3 {
 
 // Spherical potential from Phi(r), Phi'(r), Phi''(r):
 //   grad = Phi' x / r,  H_ij = (Phi'' - Phi'/r) x_i x_j / r^2 + Phi'/r delta_ij
+// In terms of the unit vector u = x/r, so that no power of 1/r beyond the
+// first is formed (at the clamped radius 1e-300 of the origin the gradient is
+// 0), with the diagonal as Phi'' u_i^2 + Phi'/r (1 - u_i^2): the form
+// (Phi'' - Phi'/r) u_i^2 + Phi'/r cancels where Phi'/r >> Phi'' (near a cusp).
 template <bool G, bool H>
 __device__ __forceinline__ void spherical_out(double x, double y, double z, double r, double P,
                                               double d1, double d2, Eval3 &o)
 {
     o.phi = P;
     if (G || H) {
-        const double ir = 1.0 / r, d1r = d1 * ir;
-        if (G) { o.g[0] = d1r * x; o.g[1] = d1r * y; o.g[2] = d1r * z; }
+        const double ir = 1.0 / r, ux = x * ir, uy = y * ir, uz = z * ir;
+        if (G) { o.g[0] = d1 * ux; o.g[1] = d1 * uy; o.g[2] = d1 * uz; }
         if (H) {
-            const double A = (d2 - d1r) * ir * ir;
-            o.h[0] = fma(A * x, x, d1r); o.h[1] = fma(A * y, y, d1r); o.h[2] = fma(A * z, z, d1r);
-            o.h[3] = A * x * y; o.h[4] = A * y * z; o.h[5] = A * x * z;
+            const double d1r = d1 * ir, A = d2 - d1r;
+            const double xx = ux * ux, yy = uy * uy, zz = uz * uz;
+            o.h[0] = fma(d2, xx, d1r * (yy + zz));
+            o.h[1] = fma(d2, yy, d1r * (xx + zz));
+            o.h[2] = fma(d2, zz, d1r * (xx + yy));
+            o.h[3] = A * ux * uy; o.h[4] = A * uy * uz; o.h[5] = A * ux * uz;
         }
     }
 }
+
+// r clamped below at 1e-300 as the reference's kernels do (`r < 1e-300 ? 1e-300
+// : r`); a NaN radius passes through, so a bad position gives NaN.
+__device__ __forceinline__ double clamp_r(double r) { return r < 1e-300 ? 1e-300 : r; }
 
 // p[]: NFW {GM, rs, M}  (reference :176-243)
 template <bool G, bool H, bool RHO>
 __device__ __forceinline__ void nfw(const double *p, double x, double y, double z, Eval3 &o)
 {
     const double GM = p[0], rs = p[1];
-    const double r = fmax(sqrt(fma(x, x, fma(y, y, z * z))), 1e-300);
-    const double L = log1p(r / rs), rr = r + rs, ir = 1.0 / r;
+    const double r = clamp_r(sqrt(fma(x, x, fma(y, y, z * z))));
+    const double u = r / rs, L = log1p(u), rr = r + rs, ir = 1.0 / r;
     const double P = -GM * L * ir;
     if (RHO) { o.rho = p[2] / (4.0 * M_PI * r * rr * rr); return; }
-    const double d1 = GM * (L * ir * ir - ir / rr);
-    const double d2 = GM * (-2.0 * L * ir * ir * ir + ir * ir / rr + (2.0 * r + rs) * ir * ir / (rr * rr));
+    double d1, d2;
+    if (u < 0.0625) {
+        // Phi' = GM/rs^2 sum_{n>=2} (-1)^n (n-1)/n u^(n-2) and
+        // Phi'' = GM/rs^3 sum_{n>=3} (-1)^n (n-1)(n-2)/n u^(n-3), 17 terms each
+        // (the first omitted term is below 1e-19 of the sum for u < 1/16):
+        // the closed forms below lose digits as 1/u (Phi') and 1/u^2 (Phi'')
+        // by cancellation, and give NaN at the origin
+        double a = 17.0 / 18.0, b = -(18.0 * 17.0) / 19.0;
+#pragma unroll
+        for (int n = 17; n >= 2; --n) a = fma(a, -u, (double)(n - 1) / n);
+#pragma unroll
+        for (int n = 18; n >= 3; --n) b = fma(b, -u, -(double)((n - 1) * (n - 2)) / n);
+        const double c = GM / (rs * rs);
+        d1 = c * a;
+        d2 = c / rs * b;
+    } else {
+        d1 = GM * (L * ir * ir - ir / rr);
+        d2 = GM * (-2.0 * L * ir * ir * ir + ir * ir / rr + (2.0 * r + rs) * ir * ir / (rr * rr));
+    }
     spherical_out<G, H>(x, y, z, r, P, d1, d2, o);
 }
 
@@ -80,7 +108,7 @@ template <bool G, bool H, bool RHO>
 __device__ __forceinline__ void hernquist(const double *p, double x, double y, double z, Eval3 &o)
 {
     const double GM = p[0], a = p[1];
-    const double r = fmax(sqrt(fma(x, x, fma(y, y, z * z))), 1e-300);
+    const double r = clamp_r(sqrt(fma(x, x, fma(y, y, z * z))));
     const double ra = r + a, ira = 1.0 / ra;
     if (RHO) { o.rho = p[2] * a / (2.0 * M_PI * r * ra * ra * ra); return; }
     const double d1 = GM * ira * ira;
@@ -93,7 +121,7 @@ template <bool G, bool H, bool RHO>
 __device__ __forceinline__ void dehnen(const double *p, double x, double y, double z, Eval3 &o)
 {
     const double GM = p[0], a = p[1], g = p[2];
-    const double r = fmax(sqrt(fma(x, x, fma(y, y, z * z))), 1e-300);
+    const double r = clamp_r(sqrt(fma(x, x, fma(y, y, z * z))));
     const double ra = r + a;
     if (RHO) {
         o.rho = (3.0 - g) * p[3] / (4.0 * M_PI * a * a * a) * pow(r / a, -g) * pow(1.0 + r / a, g - 4.0);
@@ -184,7 +212,7 @@ template <bool G, bool H, bool RHO>
 __device__ __forceinline__ void disk_ansatz(const double *p, double x, double y, double z, Eval3 &o)
 {
     const double A4 = p[0], Rd = p[1], hz = p[2], rin = p[3];
-    const double r = fmax(sqrt(fma(x, x, fma(y, y, z * z))), 1e-300), ir = 1.0 / r;
+    const double r = clamp_r(sqrt(fma(x, x, fma(y, y, z * z)))), ir = 1.0 / r;
     const double f = A4 * exp(-r / Rd - rin * ir);
     const double fo = -1.0 / Rd + rin * ir * ir;
     const double f1 = f * fo, f2 = f * (fo * fo - 2.0 * rin * ir * ir * ir);

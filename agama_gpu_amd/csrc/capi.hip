@@ -7,7 +7,10 @@ int g_grid_per_cu = 32;
 
 // 1.1.0: agx_multipole.prog_flags, the program records' packed offsets and
 // gamma, the prog_flags argument of agx_multipole_batch_eval
-extern "C" int agx_version(void) { return 10100; }
+// 1.2.0: agx_component.prog_flags (formerly padding) selects the composite
+// kernel's Multipole variant, so callers built against 1.1 must zero it or set
+// it from the table's program; NaN positions give NaN results
+extern "C" int agx_version(void) { return 10200; }
 
 extern "C" const char *agx_error_string(int code)
 {

@@ -47,6 +47,7 @@ __device__ __forceinline__ void cyl_adjoint_outer(const CSK &P, bool valid, doub
     double cph = 1.0, sph = 0.0;
     if (R > 1.0e-300) { cph = x / R; sph = y / R; }
     const double ist = pole ? 0.0 : 1.0 / st;
+    const double psg = ct < 0.0 ? -1.0 : 1.0;   // pole limit of Y / sin(theta): +-dY/dtheta
     const double q = P.r0 * ir;
     // seeds: dL/d(d Phi / d ln r), dL/d(d Phi / d theta), dL/d((d Phi / d phi) / sin theta)
     const double wPr = valid ? -(wf[0] * x + wf[1] * y + wf[2] * z) * ir * ir : 0.0;
@@ -79,13 +80,13 @@ __device__ __forceinline__ void cyl_adjoint_outer(const CSK &P, bool valid, doub
             const double k1 = -(double)(l + 1);
             // cos term W[l(l+1)+m]
             {
-                const double fphi = pole ? ((m == 1) ? -sm * dY : 0.0) : -(double)m * sm * Y * ist;
+                const double fphi = pole ? ((m == 1) ? -psg * sm * dY : 0.0) : -(double)m * sm * Y * ist;
                 const double g = rl * (cm * Y * fma(wPr, k1, wv) + wPt * cm * dY + wPf * fphi);
                 const double gs = wave_sum(g);
                 if (lane == 0) atomicAdd(gW + l * (l + 1) + m, gs);
             }
             if (m > 0) {   // sin term W[l(l+1)-m]
-                const double fphi = pole ? ((m == 1) ? cm * dY : 0.0) : (double)m * cm * Y * ist;
+                const double fphi = pole ? ((m == 1) ? psg * cm * dY : 0.0) : (double)m * cm * Y * ist;
                 const double g = rl * (sm * Y * fma(wPr, k1, wv) + wPt * sm * dY + wPf * fphi);
                 const double gs = wave_sum(g);
                 if (lane == 0) atomicAdd(gW + l * (l + 1) - m, gs);

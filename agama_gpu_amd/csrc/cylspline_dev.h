@@ -49,8 +49,11 @@ __device__ __forceinline__ CylCoords cyl_coords(const CSK &P, double x, double y
     double qR, qz;
     sqrt_rsqrt(R2 + Rs2, qR, c.jR);
     sqrt_rsqrt(fma(z, z, Rs2), qz, c.jz);
-    c.lR = fmax(agx_log(c.R + qR) - P.lnRs, 0.0);
-    c.lz = copysign(fmax(agx_log(fabs(z) + qz) - P.lnRs, 0.0), z);
+    // (compared, not fmax: a NaN coordinate stays NaN, fails every bounds test
+    // of the caller and gives a NaN result, as in the reference)
+    const double tR = agx_log(c.R + qR) - P.lnRs, tz = agx_log(fabs(z) + qz) - P.lnRs;
+    c.lR = tR < 0.0 ? 0.0 : tR;
+    c.lz = copysign(tz < 0.0 ? 0.0 : tz, z);
     return c;
 }
 
@@ -199,7 +202,7 @@ __device__ __forceinline__ void cyl_outer(const CSK &P, double x, double y, doub
     o.phi = Sv;
     if (!G && !H) return;
     SphDerivs d;
-    d.Pr = Sr; d.Pt = St; d.Pfos = pole ? Sfp : Sfn * ist;
+    d.Pr = Sr; d.Pt = St; d.Pfos = pole ? (ct < 0.0 ? -Sfp : Sfp) : Sfn * ist;   // south pole: -
     d.Prr = Srr; d.Prt = Srt; d.Prf = Srf; d.Ptt = Stt; d.Ptf = Stf; d.Pf = Sf; d.Pff = Sff;
     sph_to_cart<H>(x, y, z, ir, R, ct, st, cph, sph, d, o.g, o.h);
 }

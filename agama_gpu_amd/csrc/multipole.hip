@@ -120,11 +120,12 @@ __global__ __launch_bounds__(kBlock) void mp_sorted_kernel(MPK P, const double *
     });
 }
 
-static int check_mp(const agx_multipole *mp, const double *xyz, int64_t n)
+// The descriptor and N; the caller checks the position and output pointers
+// after its own flags, so that no invalid argument can reach a launch.
+static int check_mp(const agx_multipole *mp, int64_t n)
 {
     if (!mp) return AGX_E_NULL;
     if (n < 0) return AGX_E_SIZE;
-    if (n > 0 && !xyz) return AGX_E_NULL;
     if (!mp->poly_km || !mp->program) return AGX_E_NULL;
     if (mp->n_int < 1 || mp->n_lm < 1 || mp->n_groups < 0 || mp->n_steps < 0) return AGX_E_SIZE;
     if (mp->lmax > 32) return AGX_E_LMAX;
@@ -321,11 +322,11 @@ extern "C" int agx_multipole_eval(const agx_multipole *mp, const double *d_xyz, 
                                   double *d_phi, double *d_force, double *d_deriv, uint32_t what,
                                   void *stream)
 {
-    int rc = check_mp(mp, d_xyz, n);
+    int rc = check_mp(mp, n);
     if (rc) return rc;
     if (what == 0 || (what & ~(AGX_PHI | AGX_FORCE | AGX_DERIV))) return AGX_E_WHAT;
     if (n == 0) return AGX_OK;   // empty input: nothing to write, outputs may be NULL
-    if (((what & AGX_PHI) && !d_phi) || ((what & AGX_FORCE) && !d_force) ||
+    if (!d_xyz || ((what & AGX_PHI) && !d_phi) || ((what & AGX_FORCE) && !d_force) ||
         ((what & AGX_DERIV) && !d_deriv))
         return AGX_E_NULL;
     MPK P = make_args(mp);
@@ -352,12 +353,12 @@ extern "C" int agx_multipole_eval_sorted(const agx_multipole *mp, const double *
                                          uint32_t what, void *d_work, int64_t work_bytes,
                                          void *stream)
 {
-    int rc = check_mp(mp, d_xyz, n);
+    int rc = check_mp(mp, n);
     if (rc) return rc;
     if (what == 0 || (what & ~(AGX_PHI | AGX_FORCE | AGX_DERIV))) return AGX_E_WHAT;
     if (n > (int64_t)0x7FFFFFFF - 64 * kMaxBins || mp->n_int + 1 > kMaxBins) return AGX_E_SIZE;
     if (n == 0) return AGX_OK;
-    if (((what & AGX_PHI) && !d_phi) || ((what & AGX_FORCE) && !d_force) ||
+    if (!d_xyz || ((what & AGX_PHI) && !d_phi) || ((what & AGX_FORCE) && !d_force) ||
         ((what & AGX_DERIV) && !d_deriv))
         return AGX_E_NULL;
     if (!d_work) return AGX_E_NULL;
@@ -389,10 +390,10 @@ extern "C" int agx_multipole_eval_sorted(const agx_multipole *mp, const double *
 extern "C" int agx_multipole_density(const agx_multipole *mp, const double *d_xyz, int64_t n,
                                      double inv_4piG, double *d_rho, void *stream)
 {
-    int rc = check_mp(mp, d_xyz, n);
+    int rc = check_mp(mp, n);
     if (rc) return rc;
     if (n == 0) return AGX_OK;
-    if (!d_rho) return AGX_E_NULL;
+    if (!d_xyz || !d_rho) return AGX_E_NULL;
     MPK P = make_args(mp);
     launch_mp<ORD_DENS>(mp, P, d_xyz, n, d_rho, nullptr, nullptr, inv_4piG, (hipStream_t)stream);
     return hipGetLastError() == hipSuccess ? AGX_OK : AGX_E_LAUNCH;

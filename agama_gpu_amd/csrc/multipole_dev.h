@@ -105,6 +105,13 @@ __device__ __forceinline__ void radial_out(double x, double y, double z, double 
     }
 }
 
+// (dPhi/dphi) / sin(theta) on the z axis from the m = 1 terms' theta
+// derivatives S: Y / sin(theta) -> dY/dtheta at the north pole (cos theta = 1)
+// and -dY/dtheta at the south pole, where sin(theta) ~ pi - theta.  (The
+// reference takes +dY/dtheta at both, _multipole_potential_kernel.cu:332-334,
+// which reverses the m = 1 part of the force at points (0, 0, z < 0).)
+__device__ __forceinline__ double pole_limit(double S, double ct) { return ct < 0.0 ? -S : S; }
+
 // Per-term hook of interior(): term() sees every step of the Legendre
 // recurrence (cos/sin slab offsets, -1 when absent, P_l^m-scaled value and
 // theta derivative, m, cos/sin(m phi), 1/sin(theta), pole flag) and sums()
@@ -374,7 +381,7 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
     if (ORD == ORD_GRAD) {
         // g = (Pr u_r + Pt u_theta + Pfos u_phi) / r with the unit vectors from
         // the angles, so x, y, z are not needed after the term loop
-        const double Pt = lam * St, Pf = lam * (pole ? Sfp : Sfn * ist);
+        const double Pt = lam * St, Pf = lam * (pole ? pole_limit(Sfp, ct) : Sfn * ist);
         const double a = fma(Pr, st, Pt * ct);
         o.g[0] = ir * fma(a, cph, -Pf * sph);
         o.g[1] = ir * fma(a, sph, Pf * cph);
@@ -384,7 +391,7 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
     SphDerivs d;
     d.Pr = Pr;
     d.Pt = lam * St;
-    d.Pfos = lam * (pole ? Sfp : Sfn * ist);
+    d.Pfos = lam * (pole ? pole_limit(Sfp, ct) : Sfn * ist);
     if (ORD == ORD_HESS) {
         d.Prr = fma(lam * idl * idl, Srr, fma(2.0 * mu * idl, Sr, fma(nu, Sv, d2C0)));
         d.Ptt = lam * Stt;
@@ -396,31 +403,37 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
 }
 
 // Classification of one point: 0 origin, 1 inner law, 2 interior (k), 3 outer law.
+// Every comparison is written so that a NaN position fails it and falls through
+// to the outer law, whose value is then NaN (as in the reference, where NaN
+// reaches the spline and propagates); r^2 = +inf (|x| above about 1.3e154) is
+// the outer law too, and extrap() returns its limit there (Phi, force and
+// Hessian 0; the reference's outer law at r = inf gives the same).
 struct Where {
     int kind, k;
     double r2, r, ir, lnr;
 };
 
+__device__ __forceinline__ bool beyond_fp64(const Where &w) { return w.r2 == __builtin_inf(); }
+
 __device__ __forceinline__ Where classify(const MPK &P, double x, double y, double z)
 {
     Where w;
     w.r2 = fma(x, x, fma(y, y, z * z));
-    if (w.r2 >= 0x1p-767) {
+    if (w.r2 >= 0x1p-767) {   // (+inf gives r = NaN; beyond_fp64 catches it)
         sqrt_rsqrt(w.r2, w.r, w.ir);
-    } else {                  // r < 4e-116: the origin (ir unused) or the inner law
+    } else {                  // r < 4e-116 (the origin or the inner law), or NaN
         w.r = sqrt(w.r2);
         w.ir = 1.0 / w.r;
     }
     w.k = 0;
     w.lnr = 0.0;
-    if (!(w.r >= 1.0e-300)) { w.kind = 0; return w; }
+    if (w.r < 1.0e-300) { w.kind = 0; return w; }
     w.lnr = agx_log(w.r);
     if (w.lnr < P.logr_min) { w.kind = 1; return w; }
-    int k = (int)((w.lnr - P.logr_min) * P.inv_dlogr);
-    if (k < 0) k = 0;
-    if (k >= P.n_int) { w.kind = 3; return w; }
+    const double kd = (w.lnr - P.logr_min) * P.inv_dlogr;    // >= 0 here, or NaN
+    if (!(kd < (double)P.n_int)) { w.kind = 3; return w; }
     w.kind = 2;
-    w.k = k;
+    w.k = (int)kd;
     return w;
 }
 
@@ -442,7 +455,15 @@ __device__ __forceinline__ void extrap(const MPK &P, const Where &w, double x, d
         law(0.0, P.inner_s, P.inner_U, P.inner_W, P.inner_log, w.lnr - P.logr_min, er,
             rcp_nr(er), Pv, D, D2);
     } else {
-        if (ORD == ORD_DENS) { o.phi = 0.0; return; }
+        if (beyond_fp64(w)) {   // r = inf: every outer law decays to 0 with its derivatives
+            o.phi = 0.0;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) o.g[a] = 0.0;
+#pragma unroll
+            for (int a = 0; a < 6; ++a) o.h[a] = 0.0;
+            return;
+        }
+        if (ORD == ORD_DENS) { o.phi = 0.0 * w.r2; return; }   // 0 beyond r_max; NaN stays NaN
         law(-1.0, P.outer_s, P.outer_U, P.outer_W, P.outer_log,
             w.lnr - fma((double)P.n_int, P.dlogr, P.logr_min), w.r * P.inv_rmax, P.rmax * w.ir,
             Pv, D, D2);

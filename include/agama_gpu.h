@@ -193,7 +193,10 @@ enum {
 typedef struct agx_component {
     int32_t type;
     uint32_t prog_flags;     /* AGX_T_MULTIPOLE: the table's agx_multipole.prog_flags
-                                (0 is always correct); 0 for the other types */
+                                (0 is always correct); 0 for the other types.
+                                Padding before version 1.2.0: callers written
+                                for 1.1 must zero it (or set it as above), since
+                                the kernel now trusts it */
     double p[8];
     const void *table;
     double ampl;
@@ -398,7 +401,10 @@ int agx_leapfrog(const agx_component *d_leaves, int32_t n_leaves,
 int agx_set_option(int key, int64_t value);
 
 /* Library identification and error text.  agx_version: major * 10000 +
- * minor * 100 + patch (10100: this header). */
+ * minor * 100 + patch (10200: this header).  1.2.0: agx_component.prog_flags
+ * (formerly padding) is read; a position with a NaN coordinate gives NaN
+ * outputs, and one whose r^2 overflows (|x| above about 1.3e154) gives the
+ * outer-law limit 0 for Multipole tables. */
 int agx_version(void);
 const char *agx_error_string(int code);
 

@@ -29,13 +29,29 @@ def _radial(x, r, P, d1, d2):
     return P, g, h
 
 
+def _nfw_small(u):
+    """Phi'(r) rs^2/GM and Phi''(r) rs^3/GM of the NFW potential for u = r/rs
+    below 1/16, from the Taylor series of ln(1+u)/u (terms to u^16): the
+    closed forms lose digits as 1/u and 1/u^2 there by cancellation."""
+    d1 = sum((-1.0) ** n * (n - 1) / n * u ** (n - 2) for n in range(2, 19))
+    d2 = sum((-1.0) ** n * (n - 1) * (n - 2) / n * u ** (n - 3) for n in range(3, 20))
+    return d1, d2
+
+
 def nfw(x, M, rs):                                     # :176-243
     GM = G * M
-    r = np.maximum(np.linalg.norm(x, axis=1), 1e-300)
+    r = np.linalg.norm(x, axis=1)
+    r = np.where(r < 1e-300, 1e-300, r)                # reference clamp; NaN passes
     L = np.log1p(r / rs)
     rr = r + rs
-    d1 = GM * (L / r ** 2 - 1.0 / (r * rr))
-    d2 = GM * (-2.0 * L / r ** 3 + 1.0 / (r ** 2 * rr) + (2.0 * r + rs) / (r ** 2 * rr ** 2))
+    with np.errstate(over="ignore", invalid="ignore"):
+        d1 = GM * (L / r ** 2 - 1.0 / (r * rr))
+        d2 = GM * (-2.0 * L / r ** 3 + 1.0 / (r ** 2 * rr) + (2.0 * r + rs) / (r ** 2 * rr ** 2))
+    small = r / rs < 0.0625
+    if np.any(small):
+        s1, s2 = _nfw_small(r[small] / rs)
+        d1[small] = GM / rs ** 2 * s1
+        d2[small] = GM / rs ** 3 * s2
     P, g, h = _radial(x, r, -GM * L / r, d1, d2)
     return P, g, h, M / (4.0 * math.pi * r * rr * rr)
 

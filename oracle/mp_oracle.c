@@ -393,7 +393,10 @@ static void accum(const orc_mp *T, const term_ctx *X, acc_t *A, int ci, int l,
     A->Pff += mul * Cv * Y * d2Tm;
     A->Prf += mul * dv * Y * dTm;
     A->Ptf += mul * Cv * dY * dTm;
-    double Pos = (am == 0) ? 0.0 : (!X->near_pole ? Y / X->st : (am == 1 ? dY : 0.0));
+    /* pole limit of Y / sin(theta): +dY/dtheta at the north pole, -dY/dtheta at
+     * the south pole (the reference takes +dY/dtheta at both, kernel :332-334) */
+    double Pos = (am == 0) ? 0.0 : (!X->near_pole ? Y / X->st
+                                    : (am == 1 ? (X->ct < 0.0 ? -dY : dY) : 0.0));
     A->Pfos += mul * Cv * Pos * dTm;
     A->lap += mul * (d2v + dv - (double)(l * (l + 1)) * Cv) * Y * Tm;
 }

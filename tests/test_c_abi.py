@@ -35,13 +35,16 @@ def test_library_exports_all_declared_symbols():
 def test_version_and_error_strings():
     from agama_gpu_amd import _lib
     L = _lib.load()
-    assert L.agx_version() >= 10000
+    assert L.agx_version() == 10200
     assert b"NULL" in L.agx_error_string(-1)
     assert L.agx_error_string(12345) == b"unknown error code"
 
 
 def test_argument_validation_without_gpu():
-    """Invalid arguments are rejected on the host before any launch."""
+    """Invalid arguments are rejected on the host before any launch.  The
+    positions are NULL in every call with N > 0: the pointer checks come after
+    the descriptor and flag checks, so a check that regressed returns
+    AGX_E_NULL (-1) instead of launching on a fake address."""
     from agama_gpu_amd import _lib
     L = _lib.load()
     d = _lib.agx_multipole()
@@ -49,25 +52,25 @@ def test_argument_validation_without_gpu():
     assert L.agx_multipole_eval(C.byref(d), None, 10, None, None, None, 1, None) == -1
     d.poly_km = d.program = 8
     d.n_int, d.n_lm, d.lmax = 4, 1, 40
-    assert L.agx_multipole_eval(C.byref(d), 8, 10, 8, None, None, 1, None) == -3
+    assert L.agx_multipole_eval(C.byref(d), None, 10, None, None, None, 1, None) == -3
     d.lmax = 0
-    assert L.agx_multipole_eval(C.byref(d), 8, -1, 8, None, None, 1, None) == -2
-    assert L.agx_multipole_eval(C.byref(d), 8, 10, 8, None, None, 0, None) == -5
-    assert L.agx_multipole_eval(C.byref(d), 8, 10, None, None, None, 1, None) == -1
-    assert L.agx_multipole_eval(C.byref(d), 8, 0, 8, None, None, 1, None) == 0
-    d.prog_flags = _lib.AGX_PROG_STRIDE1
-    assert L.agx_multipole_eval(C.byref(d), 8, 0, 8, None, None, 1, None) == 0
-    d.prog_flags = _lib.AGX_PROG_COS_ONLY
-    assert L.agx_multipole_eval(C.byref(d), 8, 0, 8, None, None, 1, None) == 0
-    d.prog_flags = _lib.AGX_PROG_STRIDE2
-    assert L.agx_multipole_eval(C.byref(d), 8, 0, 8, None, None, 1, None) == 0
+    assert L.agx_multipole_eval(C.byref(d), None, -1, None, None, None, 1, None) == -2
+    assert L.agx_multipole_eval(C.byref(d), None, 10, None, None, None, 0, None) == -5
+    assert L.agx_multipole_eval(C.byref(d), None, 10, None, None, None, 1, None) == -1
+    assert L.agx_multipole_eval(C.byref(d), None, 0, None, None, None, 1, None) == 0
+    for flags in (_lib.AGX_PROG_STRIDE1, _lib.AGX_PROG_COS_ONLY, _lib.AGX_PROG_STRIDE2):
+        d.prog_flags = flags
+        assert L.agx_multipole_eval(C.byref(d), None, 0, None, None, None, 1, None) == 0
     d.prog_flags = 8                    # no such program flag
-    assert L.agx_multipole_eval(C.byref(d), 8, 10, 8, None, None, 1, None) == -5
+    assert L.agx_multipole_eval(C.byref(d), None, 10, None, None, None, 1, None) == -5
+    assert L.agx_multipole_density(C.byref(d), None, 10, 1.0, None, None) == -5
+    d.prog_flags = 0
+    assert L.agx_multipole_density(C.byref(d), None, 10, 1.0, None, None) == -1
 
 
 def test_component_flags_validation_without_gpu():
     """A Multipole component with an unknown program flag bit is rejected on
-    the host before any launch."""
+    the host before any launch (positions NULL, as above)."""
     from agama_gpu_amd import _lib
     L = _lib.load()
     comps = (_lib.agx_component * 1)()
@@ -75,9 +78,10 @@ def test_component_flags_validation_without_gpu():
     comps[0].table = 8
     comps[0].ampl = comps[0].scale = 1.0
     comps[0].prog_flags = _lib.AGX_PROG_COS_ONLY | _lib.AGX_PROG_STRIDE2
-    assert L.agx_composite_eval(comps, 1, 8, 0, 8, None, None, 1, None) == 0
+    assert L.agx_composite_eval(comps, 1, None, 0, None, None, None, 1, None) == 0
+    assert L.agx_composite_eval(comps, 1, None, 10, None, None, None, 1, None) == -1
     comps[0].prog_flags = 8                # no such program flag
-    assert L.agx_composite_eval(comps, 1, 8, 10, 8, None, None, 1, None) == -5
+    assert L.agx_composite_eval(comps, 1, None, 10, None, None, None, 1, None) == -5
 
 
 def test_descriptor_layout_matches_header():
@@ -94,7 +98,8 @@ def test_descriptor_layout_matches_header():
 
 
 def test_adjoint_workspace_validation_without_gpu():
-    """The Multipole adjoint needs its documented workspace (about 9 N bytes)."""
+    """The Multipole adjoint needs its documented workspace (about 9 N bytes);
+    positions NULL so that no failed check can launch."""
     from agama_gpu_amd import _lib
     L = _lib.load()
     n = 1000
@@ -103,8 +108,9 @@ def test_adjoint_workspace_validation_without_gpu():
     d = _lib.agx_multipole()
     d.poly_km = d.program = 8
     d.n_int, d.n_lm, d.lmax = 4, 1, 0
-    args = (C.byref(d), 8, n, 8, None, None, 8, 8)
+    args = (C.byref(d), None, n, None, None, None, 8, 8)
     assert L.agx_multipole_adjoint(*args, None, need, 0, None) == -1
     assert L.agx_multipole_adjoint(*args, 8, need - 1, 0, None) == -7
+    assert L.agx_multipole_adjoint(*args, 8, need, 0, None) == -1
     d.n_int = 128                       # n_int + 1 bins > 128
     assert L.agx_multipole_adjoint(*args, 8, need, 0, None) == -2

@@ -52,7 +52,7 @@ def high_order():
     return c, x, OracleMultipole.from_coefs(c).eval_deriv(x, 8)
 
 
-@pytest.mark.parametrize("path", [0, 1, 2, 3])
+@pytest.mark.parametrize("path", [0, 1, 2, 3, 4])
 def test_high_order_lmax32(high_order, path):
     from agama_gpu_amd import MultipolePotentialGPU, _lib
     c, x, (po, Fo, Do) = high_order
@@ -71,3 +71,37 @@ def test_high_order_lmax32(high_order, path):
     R = np.hypot(x[:, 0], x[:, 1])
     ok = (R == 0) | (R > 1e-6 * np.linalg.norm(x, axis=1))
     assert np.max(np.abs(D[ok] - Do[ok])) <= 1e-10 * np.abs(Do[ok]).max()
+
+
+def test_high_order_lmax32_sorted_auto():
+    """The benchmarked form of the lmax-32 workload: at N >= 200k the automatic
+    choice for a table too large for LDS (4.1 MB) is the interval-sorted path
+    (agx_multipole_eval_sorted); 250k points log-uniform in r over
+    [0.05, 800] kpc, evalDeriv and pot+force, against the oracle."""
+    from agama_gpu_amd import MultipolePotentialGPU, _lib
+    from agama_gpu_amd.multipole import eval_table
+    from agama_gpu_amd.workloads import high_order_coefs, loguniform_points
+    from oracle.oracle import OracleMultipole
+    c = high_order_coefs()
+    x = loguniform_points(250_000, 12, 0.05, 800.0)
+    print(f"case lmax32 sorted auto n={len(x)}", flush=True)
+    pot = MultipolePotentialGPU(c, device=DEV)
+    assert _lib.get_option(_lib.AGX_OPT_MP_PATH) == 0
+    xt = torch.as_tensor(x, device=DEV)
+    n = len(x)
+    phi = torch.empty(n, dtype=torch.float64, device=DEV)
+    F = torch.empty((n, 3), dtype=torch.float64, device=DEV)
+    D = torch.empty((n, 6), dtype=torch.float64, device=DEV)
+    work = eval_table(pot._desc, pot._data, xt, phi, F, D, _lib.AGX_PHI | _lib.AGX_FORCE |
+                      _lib.AGX_DERIV, xt.device)
+    assert work is not None                     # the sorted path ran
+    p, Fg, Dg = phi.cpu().numpy(), F.cpu().numpy(), D.cpu().numpy()
+    po, Fo, Do = OracleMultipole.from_coefs(c).eval_deriv(x)
+    assert np.max(np.abs(p - po) / np.abs(po)) < 1e-12
+    assert np.max(np.abs(Fg - Fo)) <= 1e-11 * np.abs(Fo).max()
+    R = np.hypot(x[:, 0], x[:, 1])
+    ok = (R == 0) | (R > 1e-6 * np.linalg.norm(x, axis=1))
+    assert np.max(np.abs(Dg[ok] - Do[ok])) <= 1e-10 * np.abs(Do[ok]).max()
+    p2, F2 = [t.cpu().numpy() for t in pot.eval(xt, pot=True, acc=True)]
+    assert np.max(np.abs(p2 - po) / np.abs(po)) < 1e-12
+    assert np.max(np.abs(F2 - Fo)) <= 1e-11 * np.abs(Fo).max()

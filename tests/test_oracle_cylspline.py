@@ -103,3 +103,65 @@ def test_outer_expansion_is_harmonic(orc):
     fd = np.stack([(orc.eval(x + step * e[k])[0] - orc.eval(x - step * e[k])[0]) / (2 * step[:, 0])
                    for k in range(3)], axis=1)
     assert np.max(np.abs(fd - g)) <= 1e-6 * np.abs(g).max()
+
+
+# ---- the outer PowerLaw fit against a field the mathematics fixes -------------
+def test_outer_fit_recovers_exterior_field():
+    """Tables sampled from an exterior field of solid harmonics (monopole,
+    dipole, quadrupole m = 0, +-2, octupole; tests/exact_fields.py, which
+    derives each W_lm from Agama's normalisation by hand): the boundary fit of
+    the product (its own Legendre recurrence) and of the oracle (scipy
+    sph_harm_y) both recover every W_lm to lstsq precision and give zero for
+    the absent terms.  The reference's recurrence (0 / P_(l-3) in place of
+    P_(l-2), _cylspl_potential_kernel.cu:205-223) fails this; so would a wrong
+    normalisation, a dropped Condon-Shortley sign or a transposed (l, m)."""
+    from exact_fields import exterior_W, exterior_coefs
+    from agama_gpu_amd.cylspline_prep import outer_asymptote
+    c = exterior_coefs()
+    W, r0 = outer_asymptote(c.R_grid, c.z_grid, c.phi, c.mmax)
+    assert r0 == 60.0
+    We = exterior_W(r0)
+    assert np.max(np.abs(W - We)) < 1e-12 * np.abs(We).max()
+    Wo, r0o = OracleCylSpline._fit(c.R_grid, c.z_grid, c.phi, c.mmax)
+    assert r0o == r0 and np.max(np.abs(Wo - We)) < 1e-12 * np.abs(We).max()
+
+
+def test_rscale_and_outer_values_follow():
+    """With the fitted W: Rscale = -Mtot / Phi0 with Mtot = -W_00 r0 2 sqrt(pi)
+    and Phi0 the m = 0 node at R = 0, iz = (nz+1)//2 (gpu_potential.py:
+    1188-1207); outside the grid box the oracle equals the generating field,
+    and across the box's boundary nodes the outer expansion continues the
+    inner spline (both equal the node value there)."""
+    from exact_fields import exterior_W, exterior_coefs, exterior_phi
+    c = exterior_coefs()
+    d = build_cylspline_data(c)
+    o = OracleCylSpline(c)
+    We = exterior_W(60.0)
+    Phi0 = c.phi[0][0, (len(c.z_grid) + 1) // 2]
+    want = -(-We[0] * 60.0 * 2 * np.sqrt(np.pi)) / Phi0
+    assert d["Rscale"] == pytest.approx(want, rel=1e-12) and o.Rscale == pytest.approx(want, rel=1e-12)
+    rng = np.random.default_rng(4)
+    n = 400
+    r = np.exp(rng.uniform(np.log(110.0), np.log(2000.0), n))
+    ct = rng.uniform(-1, 1, n)
+    ph = rng.uniform(0, 2 * np.pi, n)
+    st = np.sqrt(1 - ct * ct)
+    x = np.stack([r * st * np.cos(ph), r * st * np.sin(ph), r * ct], axis=1)
+    P, g, H, outer = o.eval(x)
+    assert outer.all()
+    ex = exterior_phi(x[:, 0], x[:, 1], x[:, 2])
+    assert np.max(np.abs(P - ex) / np.abs(ex)) < 1e-12
+    # gradient of the closed form by complex-step differentiation
+    ge = np.stack([np.imag(exterior_phi(*(x + 1e-30j * e).T)) / 1e-30 for e in np.eye(3)], axis=1)
+    assert np.max(np.abs(g - ge)) < 1e-11 * np.abs(ge).max()
+    # boundary nodes (top edge and outer R edge), just inside vs just outside
+    R, z = c.R_grid, c.z_grid
+    pts = [(R[i], z[-1]) for i in range(1, len(R) - 1)] + [(R[-1], z[j]) for j in range(len(z))]
+    pts = np.array(pts)
+    ph = rng.uniform(0, 2 * np.pi, len(pts))
+    base = np.stack([pts[:, 0] * np.cos(ph), pts[:, 0] * np.sin(ph), pts[:, 1]], axis=1)
+    for eps in (1e-9,):
+        pin, _, _, oin = o.eval(base * (1 - eps))
+        pout, _, _, oout = o.eval(base * (1 + eps))
+        assert not oin.any() and oout.all()
+        assert np.max(np.abs(pin - pout) / np.abs(pout)) < 1e-8

@@ -210,12 +210,95 @@ def test_origin_and_pole_branches():
     d = build_multipole_data(c)
     p, F = o.pot_force(np.zeros((1, 3)))
     assert p[0] == pytest.approx(d["inner_W"], rel=1e-15) and np.all(F == 0)
-    # approach to the z axis is continuous (reference tests/test_gpu_audit.py:176-203)
-    for z in (5.0, -5.0):
-        near = np.array([[1e-7, 0.0, z], [0.0, 1e-7, z]])
+    # approach to the z axis is continuous, every force component, at both
+    # poles (reference tests/test_gpu_audit.py:176-203): the m = 1 terms' limit
+    # of Y / sin(theta) is +dY/dtheta at the north pole and -dY/dtheta at the
+    # south pole (the reference's +dY/dtheta at both reverses the m = 1 part of
+    # F_x, F_y at (0, 0, z < 0))
+    for z in (5.0, -5.0, 40.0, -40.0):
+        ph = np.linspace(0.0, 2 * np.pi, 8, endpoint=False)
+        near = np.stack([1e-7 * np.cos(ph), 1e-7 * np.sin(ph), z + 0 * ph], axis=1)
         on = np.array([[0.0, 0.0, z]])
         pn, Fn = o.pot_force(near)
         po, Fo = o.pot_force(on)
         assert np.max(np.abs(pn - po[0])) < 1e-9 * abs(po[0])
-        # the z component and the x component (through d/dtheta) are continuous
-        assert np.max(np.abs(Fn[:, 2] - Fo[0, 2])) < 1e-6 * np.abs(Fo).max()
+        assert np.max(np.abs(Fn - Fo[0])) < 1e-6 * np.abs(Fo).max(), z
+        assert np.abs(Fo[0, :2]).max() > 1e-3 * abs(Fo[0, 2])     # m = 1 terms present
+
+
+# ---- extrapolation laws: tables whose continuation beyond the grid is fixed ----
+LAW_CASES = ["inner_log", "inner_core", "outer_log", "kepler", "outer_power"]
+
+
+@pytest.mark.parametrize("name", LAW_CASES)
+def test_extrapolation_law_reproduced(name):
+    """A monopole table sampled from one of the l=0 extrapolation laws is
+    continued by exactly that law beyond the grid (closed forms in
+    tests/exact_fields.py): value, force and the radial Hessian
+    (D2 - 2D) x x / r^4 + D / r^2, with D, D2 the ln r derivatives.  The
+    reference's coefficient (D2 - D) fails the Hessian check; a Lambert-W root
+    other than s = v, or a missing s = 2 override, fails the value check."""
+    from exact_fields import radial_exact, radial_points, radial_table
+    o = OracleMultipole.from_coefs(radial_table(name))
+    x = radial_points(name, 300, 1)
+    p, F, D = o.eval_deriv(x)
+    pe, Fe, De = radial_exact(name, x)
+    # relative to |Phi| + |r dPhi/dr| (the log laws pass through Phi = 0)
+    scale = np.abs(pe) + np.linalg.norm(Fe, axis=1) * np.linalg.norm(x, axis=1)
+    assert np.max(np.abs(p - pe) / scale) < 1e-12
+    assert np.max(np.abs(F - Fe)) < 1e-11 * np.abs(Fe).max()
+    assert np.max(np.abs(D - De)) < 1e-10 * np.abs(De).max()
+    # density: Laplacian / (4 pi G) of the law; 0 beyond r_max (reference :851)
+    rho = o.density(x)
+    if name.startswith("inner"):
+        assert np.max(np.abs(rho + (De[:, 0] + De[:, 1] + De[:, 2]) * INV_4PIG)) \
+            < 1e-10 * np.abs(De[:, :3].sum(1) * INV_4PIG).max()
+    else:
+        assert np.all(rho == 0.0)
+
+
+def test_law_modes_chosen():
+    """The law each exact table selects: log (s = v), the s = 2 core, or the
+    Keplerian fallback."""
+    from exact_fields import radial_table
+    want = {"inner_log": ("inner_mode", "log"), "inner_core": ("inner_mode", "s2"),
+            "outer_log": ("outer_mode", "log"), "kepler": ("outer_mode", "kepler"),
+            "outer_power": ("outer_mode", "log")}
+    for name, (k, v) in want.items():
+        assert build_multipole_data(radial_table(name))[k] == v, name
+
+
+@pytest.mark.parametrize("fname", FILES)
+@pytest.mark.parametrize("region", ["inner", "outer"])
+def test_calculus_beyond_grid(fname, region):
+    """force = -grad Phi and deriv = -grad force by 5-point central
+    differences (step 1e-3 r, truncation ~1e-12) below r_min and above r_max,
+    where the l=0 laws apply (the interior test above samples r in [0.3, 300]
+    only).  Inside r_min the potential is nearly constant (the FIRE table's
+    s = 2 core: |Phi| ~ 1e5, |F| ~ 1 at r ~ 1e-4), which bounds the
+    differences' round-off; r >= 0.1 r_min keeps it below 1e-7."""
+    c = read_coefs(FIXTURES / fname)
+    o = OracleMultipole.from_coefs(c)
+    rng = np.random.default_rng(8)
+    u = unit_vectors(200, 9)
+    if region == "inner":
+        r = c.R_grid[0] * np.exp(rng.uniform(np.log(0.1), np.log(0.9), 200))
+    else:
+        r = c.R_grid[-1] * np.exp(rng.uniform(np.log(1.1), np.log(100.0), 200))
+    x = u * r[:, None]
+    p, F, D = o.eval_deriv(x)
+    h = 1e-3 * r[:, None]
+
+    def d5(f, e):
+        d = (8 * (f(x + h * e) - f(x - h * e)) - (f(x + 2 * h * e) - f(x - 2 * h * e)))
+        return d / (12 * (h[:, 0] if d.ndim == 1 else h))
+
+    g = np.stack([d5(o.potential, e) for e in np.eye(3)], axis=1)
+    assert np.max(np.abs(g + F) / np.abs(F).max(1, keepdims=True)) < 1e-7
+    cols = [(0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (0, 2)]
+    dF = [d5(lambda y: o.pot_force(y)[1], e) for e in np.eye(3)]
+    for slot, (i, j) in enumerate(cols):
+        assert np.max(np.abs(dF[j][:, i] - D[:, slot]) / np.abs(D).max(1)) < 1e-7, slot
+    # radial: the force is parallel to x
+    cross = np.cross(F, x)
+    assert np.max(np.linalg.norm(cross, axis=1) / (np.linalg.norm(F, axis=1) * r)) < 1e-12
