@@ -83,6 +83,8 @@ AGX_OPT_MP_BIN_GRID = 7
 AGX_OPT_GRID_PER_CU = 8
 AGX_OPT_SORT_GRID = 9
 AGX_BATCH_MAX_ITEM = 16384
+AGX_NOALLOC_MAX_N = 11_000_000
+AGX_E_SCRATCH = -8
 AGX_BATCH_LANE, AGX_BATCH_LDS = 0, 1
 
 # agx_sched_entry as a numpy record (int32 leaf, pad; ampl, scale, center[3])
@@ -130,6 +132,9 @@ _SIGNATURES = {
     "agx_multipole_eval_sorted": (C.c_int, [C.POINTER(agx_multipole), C.c_void_p, C.c_int64,
                                             C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
                                             C.c_void_p, C.c_int64, C.c_void_p]),
+    "agx_multipole_eval_sorted_noalloc": (C.c_int, [C.POINTER(agx_multipole), C.c_void_p,
+                                                    C.c_int64, C.c_void_p, C.c_void_p,
+                                                    C.c_void_p, C.c_uint32, C.c_void_p]),
     "agx_multipole_build": (C.c_int, [C.POINTER(agx_multipole_source), C.c_void_p, C.c_void_p,
                                       C.c_void_p]),
     "agx_multipole_build_adjoint": (C.c_int, [C.POINTER(agx_multipole_source), C.c_void_p,

@@ -23,6 +23,9 @@ extern "C" const char *agx_error_string(int code)
     case AGX_E_WHAT: return "no output requested, or an unknown output flag";
     case AGX_E_TYPE: return "unknown potential component type";
     case AGX_E_WORKSPACE: return "workspace too small";
+    case AGX_E_SCRATCH:
+        return "N exceeds the static scratch of agx_multipole_eval_sorted_noalloc "
+               "(AGX_NOALLOC_MAX_N points)";
     default: return "unknown error code";
     }
 }

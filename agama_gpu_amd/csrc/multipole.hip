@@ -1,4 +1,5 @@
 // Multipole kernels and C-ABI entry points (gfx950, FP64).
+#include <mutex>
 #include "multipole_dev.h"
 #include "sort_dev.h"
 
@@ -348,23 +349,15 @@ extern "C" int64_t agx_multipole_work_size(int64_t n)
     return n < 0 ? 0 : sort_work_size(n, kMaxBins);
 }
 
-extern "C" int agx_multipole_eval_sorted(const agx_multipole *mp, const double *d_xyz, int64_t n,
-                                         double *d_phi, double *d_force, double *d_deriv,
-                                         uint32_t what, void *d_work, int64_t work_bytes,
-                                         void *stream)
+// The interval-sorted evaluation with its workspace at d_work (at least
+// sort_work_size(n, kMaxBins) bytes); arguments already validated.  Every
+// workspace word that the kernels read is written by them first (the sort's
+// init kernel clears the histogram and the queue), so no clearing is needed.
+static int run_sorted(const agx_multipole *mp, const double *d_xyz, int64_t n, double *d_phi,
+                      double *d_force, double *d_deriv, uint32_t what, void *d_work,
+                      hipStream_t s)
 {
-    int rc = check_mp(mp, n);
-    if (rc) return rc;
-    if (what == 0 || (what & ~(AGX_PHI | AGX_FORCE | AGX_DERIV))) return AGX_E_WHAT;
-    if (n > (int64_t)0x7FFFFFFF - 64 * kMaxBins || mp->n_int + 1 > kMaxBins) return AGX_E_SIZE;
-    if (n == 0) return AGX_OK;
-    if (!d_xyz || ((what & AGX_PHI) && !d_phi) || ((what & AGX_FORCE) && !d_force) ||
-        ((what & AGX_DERIV) && !d_deriv))
-        return AGX_E_NULL;
-    if (!d_work) return AGX_E_NULL;
-    if (work_bytes < sort_work_size(n, kMaxBins)) return AGX_E_WORKSPACE;
     MPK P = make_args(mp);
-    hipStream_t s = (hipStream_t)stream;
     const SortWork W = sort_work(d_work, n, kMaxBins);
     const int nb = mp->n_int + 1;
     sort_launch(MpBinKey{P, d_xyz}, n, nb, W, s);
@@ -385,6 +378,81 @@ extern "C" int agx_multipole_eval_sorted(const agx_multipole *mp, const double *
         hipLaunchKernelGGL(mp_sorted_kernel<ORD_POT>, dim3(grid), dim3(kBlock), 0, s, P, d_xyz, phi,
                            nullptr, nullptr, 0.0, W, run_len);
     return hipGetLastError() == hipSuccess ? AGX_OK : AGX_E_LAUNCH;
+}
+
+// Validation shared by the two sorted entry points: the descriptor, `what`
+// and N, then (for N > 0) the pointers.
+static int check_sorted(const agx_multipole *mp, int64_t n, uint32_t what)
+{
+    int rc = check_mp(mp, n);
+    if (rc) return rc;
+    if (what == 0 || (what & ~(AGX_PHI | AGX_FORCE | AGX_DERIV))) return AGX_E_WHAT;
+    if (n > (int64_t)0x7FFFFFFF - 64 * kMaxBins || mp->n_int + 1 > kMaxBins) return AGX_E_SIZE;
+    return AGX_OK;
+}
+
+static int check_sorted_ptrs(const double *d_xyz, double *d_phi, double *d_force,
+                             double *d_deriv, uint32_t what)
+{
+    if (!d_xyz || ((what & AGX_PHI) && !d_phi) || ((what & AGX_FORCE) && !d_force) ||
+        ((what & AGX_DERIV) && !d_deriv))
+        return AGX_E_NULL;
+    return AGX_OK;
+}
+
+// Device scratch of agx_multipole_eval_sorted_noalloc: one copy per GPU,
+// resident for the life of the process.  Host code never uses this name as an
+// address (on the host it is only a stand-in for the symbol); the device
+// address of each GPU's copy comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) unsigned char scratch[AGX_NOALLOC_SCRATCH_BYTES];
+static_assert(sort_work_size(AGX_NOALLOC_MAX_N, kMaxBins) <= AGX_NOALLOC_SCRATCH_BYTES,
+              "AGX_NOALLOC_MAX_N points must fit the static scratch");
+
+constexpr int kMaxDevices = 64;
+static void *g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mutex;
+
+// This device's copy of `scratch`, or NULL if its address cannot be had.
+static void *scratch_for_current_device()
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
+extern "C" int agx_multipole_eval_sorted(const agx_multipole *mp, const double *d_xyz, int64_t n,
+                                         double *d_phi, double *d_force, double *d_deriv,
+                                         uint32_t what, void *d_work, int64_t work_bytes,
+                                         void *stream)
+{
+    int rc = check_sorted(mp, n, what);
+    if (rc || n == 0) return rc;
+    if (work_bytes < sort_work_size(n, kMaxBins)) return AGX_E_WORKSPACE;
+    rc = check_sorted_ptrs(d_xyz, d_phi, d_force, d_deriv, what);
+    if (rc) return rc;
+    if (!d_work) return AGX_E_NULL;
+    return run_sorted(mp, d_xyz, n, d_phi, d_force, d_deriv, what, d_work, (hipStream_t)stream);
+}
+
+extern "C" int agx_multipole_eval_sorted_noalloc(const agx_multipole *mp, const double *d_xyz,
+                                                 int64_t n, double *d_phi, double *d_force,
+                                                 double *d_deriv, uint32_t what, void *stream)
+{
+    int rc = check_sorted(mp, n, what);
+    if (rc || n == 0) return rc;
+    if (n > AGX_NOALLOC_MAX_N || sort_work_size(n, kMaxBins) > AGX_NOALLOC_SCRATCH_BYTES)
+        return AGX_E_SCRATCH;
+    rc = check_sorted_ptrs(d_xyz, d_phi, d_force, d_deriv, what);
+    if (rc) return rc;
+    void *work = scratch_for_current_device();
+    if (!work) return AGX_E_LAUNCH;
+    return run_sorted(mp, d_xyz, n, d_phi, d_force, d_deriv, what, work, (hipStream_t)stream);
 }
 
 extern "C" int agx_multipole_density(const agx_multipole *mp, const double *d_xyz, int64_t n,

@@ -25,9 +25,9 @@ struct SortWork {
     int *idx;        // [n + 64 nb] particle of each slot
 };
 
-inline int64_t sort_header_bytes(int nb) { return ((int64_t)4 * (3 * nb + 2) + 255) & ~(int64_t)255; }
+constexpr int64_t sort_header_bytes(int nb) { return ((int64_t)4 * (3 * nb + 2) + 255) & ~(int64_t)255; }
 
-inline int64_t sort_work_size(int64_t n, int nb)
+constexpr int64_t sort_work_size(int64_t n, int nb)
 {
     return sort_header_bytes(nb) + ((2 * n + 255) & ~(int64_t)255) + 4 * (n + 64 * (int64_t)nb);
 }

@@ -34,7 +34,8 @@ enum {
     AGX_E_LAUNCH = -4,      /* kernel launch failed (hipGetLastError)          */
     AGX_E_WHAT = -5,        /* no output requested / unknown flag              */
     AGX_E_TYPE = -6,        /* unknown component type in a composite           */
-    AGX_E_WORKSPACE = -7    /* workspace too small                             */
+    AGX_E_WORKSPACE = -7,   /* workspace too small                             */
+    AGX_E_SCRATCH = -8      /* N beyond the static scratch of a no-alloc call  */
 };
 
 /* output selection flags */
@@ -122,6 +123,22 @@ int agx_multipole_eval(const agx_multipole *mp, const double *d_xyz, int64_t n,
 int agx_multipole_eval_sorted(const agx_multipole *mp, const double *d_xyz, int64_t n,
                               double *d_phi, double *d_force, double *d_deriv, uint32_t what,
                               void *d_work, int64_t work_bytes, void *stream);
+
+/* As agx_multipole_eval_sorted, with the sort in a static device array of the
+ * library (AGX_NOALLOC_SCRATCH_BYTES per GPU, reserved when the library's code
+ * is loaded on that GPU and held until the process ends) instead of a caller
+ * workspace: the call allocates nothing (no hipMalloc, no allocating library
+ * routine), for callers that cannot allocate -- code under HIP graph
+ * capture, serving loops, harnesses that forbid allocation.  No counterpart
+ * in the reference.  N <= AGX_NOALLOC_MAX_N (AGX_E_SCRATCH otherwise).  Every
+ * call on one GPU uses that GPU's one array: calls that overlap on a GPU (two
+ * streams, two host threads) race and must be serialised by the caller; the
+ * array keeps the last call's sort, and each call rewrites what it reads. */
+#define AGX_NOALLOC_SCRATCH_BYTES (64ll << 20)
+#define AGX_NOALLOC_MAX_N 11000000ll
+int agx_multipole_eval_sorted_noalloc(const agx_multipole *mp, const double *d_xyz, int64_t n,
+                                      double *d_phi, double *d_force, double *d_deriv,
+                                      uint32_t what, void *stream);
 
 /* Density rho = Laplacian(Phi) / (4 pi G) at N points.
  * Replaces multipole_density_kernel (_multipole_potential_kernel.cu:809-958)

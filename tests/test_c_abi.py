@@ -114,3 +114,20 @@ def test_adjoint_workspace_validation_without_gpu():
     assert L.agx_multipole_adjoint(*args, 8, need, 0, None) == -1
     d.n_int = 128                       # n_int + 1 bins > 128
     assert L.agx_multipole_adjoint(*args, 8, need, 0, None) == -2
+
+
+def test_noalloc_validation_without_gpu():
+    """agx_multipole_eval_sorted_noalloc rejects N beyond its static scratch
+    (AGX_E_SCRATCH) before its pointer checks, and NULL positions before any
+    device call (both with N > 0 and NULL positions: nothing can launch)."""
+    from agama_gpu_amd import _lib
+    L = _lib.load()
+    d = _lib.agx_multipole()
+    d.poly_km = d.program = 8
+    d.n_int, d.n_lm, d.lmax = 4, 1, 0
+    f = L.agx_multipole_eval_sorted_noalloc
+    assert f(C.byref(d), None, _lib.AGX_NOALLOC_MAX_N + 1, None, None, None, 1, None) == -8
+    assert f(C.byref(d), None, _lib.AGX_NOALLOC_MAX_N, None, None, None, 1, None) == -1
+    assert f(C.byref(d), None, 0, None, None, None, 1, None) == 0
+    assert f(C.byref(d), None, 10, None, None, None, 0, None) == -5
+    assert b"static scratch" in L.agx_error_string(-8)
