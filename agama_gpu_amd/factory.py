@@ -5,11 +5,13 @@ Same entry points and semantics as the reference (gpu_potential.py:
 2154-2304, ``_is_potential_ini`` 2140-2151, ``_coerce`` 2127-2137,
 ``_apply_modifiers`` 2097-2124, type map 2060-2094).
 
-Types the reference builds by calling Agama on the CPU (Disk, Spheroid, King,
-triaxial Dehnen; gpu_potential.py:1935-2052) need Agama, which is not
-available here: triaxial Dehnen and Spheroid are built instead by this
-package's own Poisson solver (expansion.py) as Multipole tables; Disk and King
-raise with an explanation.
+Types the reference builds by calling Agama on the CPU (Spheroid, King, Disk,
+triaxial Dehnen; gpu_potential.py:1935-2052) are built here by this package's
+own code instead (expansion.py), since Agama is not available: Spheroid and
+triaxial Dehnen as Multipole tables from their densities by the Poisson
+solver, King as a monopole table from its Poisson ODE, Disk as DiskAnsatz plus
+an axisymmetric lmax-32 Multipole of the residual density (Agama's own
+construction, reference gpu_potential.py:1984-2052).
 """
 from __future__ import annotations
 
@@ -91,19 +93,68 @@ def _dehnen(**kw):
     return MultipolePotentialGPU(coefs)
 
 
-def _needs_agama(name):
-    def f(**kw):
-        raise NotImplementedError(
-            f"type={name!r} is built by Agama's CPU code in the reference "
-            "(gpu_potential.py:1935-2052); Agama is not available.  Build its Multipole "
-            "table elsewhere and pass the coefficient file instead.")
-    return f
+def _grid_kw(kl: dict, lmax: int, nr: int) -> dict:
+    out = dict(lmax=int(kl.get("lmax", lmax)), gridSizeR=int(kl.get("gridsizer", nr)))
+    if "rmin" in kl:
+        out["rmin"] = float(kl["rmin"])
+    if "rmax" in kl:
+        out["rmax"] = float(kl["rmax"])
+    return out
+
+
+def _spheroid(**kw):
+    """Agama's Spheroid (reference _build_spheroid_gpu, gpu_potential.py:1935-1939):
+    densityNorm or mass, scaleRadius, alpha, beta, gamma, outerCutoffRadius,
+    cutoffStrength, axisRatioY, axisRatioZ; lmax (default 8), gridSizeR (50),
+    rMin / rMax of the table."""
+    from .expansion import spheroid_coefs
+    kl = _kl(kw)
+    names = {"densitynorm": "densityNorm", "scaleradius": "scaleRadius", "alpha": "alpha",
+             "beta": "beta", "gamma": "gamma", "outercutoffradius": "outerCutoffRadius",
+             "cutoffstrength": "cutoffStrength", "axisratioy": "axisRatioY",
+             "axisratioz": "axisRatioZ"}
+    args = {v: float(kl[k]) for k, v in names.items() if k in kl}
+    if "mass" in kl:
+        args["mass"] = float(kl["mass"])
+    return MultipolePotentialGPU(spheroid_coefs(**_grid_kw(kl, 8, 50), **args))
+
+
+def _king(**kw):
+    """King (lowered isothermal) sphere (reference _build_king_gpu,
+    gpu_potential.py:1942-1945): mass, scaleRadius (the King radius), W0,
+    trunc (1: King 1966); gridSizeR of the monopole table (default 60)."""
+    from .expansion import king_coefs
+    kl = _kl(kw)
+    if "w0" not in kl:
+        raise ValueError("King potential needs W0 (the dimensionless central potential)")
+    return MultipolePotentialGPU(king_coefs(mass=float(kl.get("mass", 1.0)),
+                                            scaleRadius=float(kl.get("scaleradius", 1.0)),
+                                            W0=float(kl["w0"]), trunc=float(kl.get("trunc", 1.0)),
+                                            gridSizeR=int(kl.get("gridsizer", 60))))
+
+
+def _disk(**kw):
+    """Exponential disk as DiskAnsatz + axisymmetric Multipole (lmax 32) of the
+    residual density (reference _build_disk_gpu, gpu_potential.py:1984-2052):
+    surfaceDensity (or densityNorm, or mass), scaleRadius, scaleHeight
+    (> 0 exponential, < 0 isothermal, 0 thin), innerCutoffRadius; lmax,
+    gridSizeR, rMin, rMax of the Multipole part."""
+    from .analytic import DiskAnsatzPotentialGPU
+    from .expansion import disk_potential_parts
+    kl = _kl(kw)
+    sig = kl.get("surfacedensity", kl.get("densitynorm"))
+    ansatz, coefs = disk_potential_parts(
+        surfaceDensity=None if sig is None else float(sig),
+        mass=float(kl["mass"]) if "mass" in kl else None,
+        scaleRadius=float(kl.get("scaleradius", 1.0)), scaleHeight=float(kl.get("scaleheight", 0.1)),
+        innerCutoffRadius=float(kl.get("innercutoffradius", 0.0)),
+        sersicIndex=float(kl.get("sersicindex", 1.0)), **_grid_kw(kl, 32, 50))
+    return CompositePotentialGPU([DiskAnsatzPotentialGPU(**ansatz), MultipolePotentialGPU(coefs)])
 
 
 def type_map() -> dict:
     m = dict(ANALYTIC_TYPES)
-    m.update({"dehnen": _dehnen, "disk": _needs_agama("Disk"), "king": _needs_agama("King"),
-              "spheroid": _needs_agama("Spheroid")})
+    m.update({"dehnen": _dehnen, "disk": _disk, "king": _king, "spheroid": _spheroid})
     return m
 
 
