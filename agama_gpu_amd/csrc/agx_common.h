@@ -315,6 +315,9 @@ constexpr double G_AGAMA_D = 4.30091727067736e-06;
 // Workgroups per CU of the grid-stride launches (agx_set_option
 // AGX_OPT_GRID_PER_CU; defined in capi.hip).
 extern int g_grid_per_cu;
+// AGX_OPT_CYL_PATH of agx_set_option (defined in cylspline.hip).
+int cyl_set_path(int64_t value);
+int cyl_path();
 
 // Grid size for a grid-stride launch over n items: enough blocks to fill
 // 256 CUs several times over (oversubscribed, so that workgroups finishing

@@ -22,11 +22,23 @@ constexpr int kMaxComponents = AGX_MAX_COMPONENTS;
 
 struct CompositeArgs {
     int n;
+    int defer;          // the CylSpline leaf whose points outside its grid box are
+                        // deferred (see composite_kernel), or -1
     agx_component c[kMaxComponents];
 };
 
 // STRIDE and SIN as for interior(), from the program flags every Multipole
 // leaf's table satisfies.
+//
+// With A.defer >= 0 the points outside that CylSpline leaf's grid box are
+// deferred: a lane adds every other leaf (and the spline part when its point
+// is inside the box) and, for a point outside, appends the point's index and
+// partial sums to a workgroup queue in LDS instead of evaluating the outer
+// expansion; whenever the queue holds a workgroup's worth (kBlock) of points,
+// every lane takes one, adds the outer expansion and writes the outputs, and
+// the rest are taken at the end.  The outer expansion (lmax 8) then runs in
+// full wavefronts instead of in nearly every wavefront (almost every one of
+// them holds a point outside the box).  Every output is written once.
 template <int ORD, int STRIDE, bool SIN>
 __global__ __launch_bounds__(kBlock) void composite_kernel(CompositeArgs A,
                                                            const double *__restrict__ xyz,
@@ -36,19 +48,76 @@ __global__ __launch_bounds__(kBlock) void composite_kernel(CompositeArgs A,
 {
     constexpr bool G = ORD == ORD_GRAD || ORD == ORD_HESS;
     constexpr bool H = ORD == ORD_HESS;
-    for_each_point<kBlock>(xyz, n, [&](int64_t i, double x0, double y0, double z0) {
-        double P = 0.0, g[3] = {0.0, 0.0, 0.0}, h[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int c = 0; c < A.n; ++c) {
-            const agx_component &C = A.c[c];
-            add_leaf<ORD, true, STRIDE, SIN>(C.type, C.p, C.table, C.ampl, C.scale, C.center, x0,
-                                             y0, z0, P, g, h);
-        }
-        if (phi) __builtin_nontemporal_store(P, phi + i);        // streaming: written once
+    constexpr int NQ = 2 * kBlock;            // the queue never holds 2 kBlock points
+    __shared__ int64_t q_idx[NQ];
+    __shared__ double q_sum[(1 + (G ? 3 : 0) + (H ? 6 : 0)) * NQ];
+    __shared__ int queued;
+    const int t = threadIdx.x;
+    const int dc = ORD == ORD_DENS ? -1 : A.defer;   // the outer expansion has no density
+    auto store = [&](int64_t i, double P, const double g[3], const double h[6]) {
+        if (phi) __builtin_nontemporal_store(P, phi + i);    // streaming: written once
         if (G && force)
             for (int k = 0; k < 3; ++k) __builtin_nontemporal_store(-g[k], force + 3 * i + k);
         if (H && deriv)
             for (int k = 0; k < 6; ++k) __builtin_nontemporal_store(-h[k], deriv + 6 * i + k);
-    });
+    };
+    // the deferred leaf's outer expansion for queue entry e, then the outputs
+    auto finish = [&](int e) {
+        const int64_t j = q_idx[e];
+        double P = q_sum[e], g[3] = {0.0, 0.0, 0.0}, h[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (G)
+            for (int k = 0; k < 3; ++k) g[k] = q_sum[(1 + k) * NQ + e];
+        if (H)
+            for (int k = 0; k < 6; ++k) h[k] = q_sum[(4 + k) * NQ + e];
+        cyl_leaf_outside<ORD>(A.c[dc], xyz[3 * j], xyz[3 * j + 1], xyz[3 * j + 2], P, g, h);
+        store(j, P, g, h);
+    };
+    if (t == 0) queued = 0;
+    __syncthreads();
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n; base += stride) {
+        const int64_t i = base + t;
+        if (i < n) {
+            const double x0 = xyz[3 * i], y0 = xyz[3 * i + 1], z0 = xyz[3 * i + 2];
+            double P = 0.0, g[3] = {0.0, 0.0, 0.0}, h[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            bool later = false;
+            for (int c = 0; c < A.n; ++c) {
+                const agx_component &C = A.c[c];
+                if (c == dc)
+                    later = !cyl_leaf_inside<ORD>(C, x0, y0, z0, P, g, h);
+                else
+                    add_leaf<ORD, true, STRIDE, SIN>(C.type, C.p, C.table, C.ampl, C.scale,
+                                                     C.center, x0, y0, z0, P, g, h);
+            }
+            if (later) {
+                const int e = atomicAdd(&queued, 1);
+                q_idx[e] = i;
+                q_sum[e] = P;
+                if (G)
+                    for (int k = 0; k < 3; ++k) q_sum[(1 + k) * NQ + e] = g[k];
+                if (H)
+                    for (int k = 0; k < 6; ++k) q_sum[(4 + k) * NQ + e] = h[k];
+            } else {
+                store(i, P, g, h);
+            }
+        }
+        if (dc < 0) continue;                    // nothing is ever queued: no barriers
+        __syncthreads();
+        const int qn = queued;
+        // every lane holds the same qn before any lane can add to the queue
+        // again (the branch below has barriers)
+        __syncthreads();
+        if (qn >= kBlock) {
+            finish(qn - kBlock + t);
+            __syncthreads();                     // every lane has read its entry
+            if (t == 0) queued = qn - kBlock;
+            __syncthreads();
+        }
+    }
+    if (dc >= 0) {
+        const int qn = queued;
+        for (int e = t; e < qn; e += kBlock) finish(e);
+    }
 }
 
 // Sums whose leaves are all analytic types, which are bound by HBM.  Each
@@ -127,6 +196,7 @@ static int check_components(const agx_component *comps, int n_comps, CompositeAr
     if (n_comps < 1 || n_comps > kMaxComponents) return AGX_E_SIZE;
     if (!comps) return AGX_E_NULL;
     A.n = n_comps;
+    A.defer = -1;
     any_expansion = false;
     mp_flags = kProgFlags;                 // AND over the Multipole leaves
     for (int c = 0; c < n_comps; ++c) {
@@ -140,6 +210,7 @@ static int check_components(const agx_component *comps, int n_comps, CompositeAr
             if (C.prog_flags & ~kProgFlags) return AGX_E_WHAT;
             mp_flags &= C.prog_flags;
         }
+        if (C.type == AGX_T_CYLSPLINE && A.defer < 0 && cyl_path() == 0) A.defer = c;
         A.c[c] = C;
     }
     return AGX_OK;

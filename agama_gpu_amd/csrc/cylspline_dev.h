@@ -207,20 +207,48 @@ __device__ __forceinline__ void cyl_outer(const CSK &P, double x, double y, doub
     sph_to_cart<H>(x, y, z, ir, R, ct, st, cph, sph, d, o.g, o.h);
 }
 
+// Outside the grid box in scaled coordinates (a NaN coordinate is inside:
+// every comparison fails, and the spline then gives NaN).
+__device__ __forceinline__ bool cyl_outside(const CSK &P, const CylCoords &cc)
+{
+    return cc.lR < cld(P.lR, 0) || cc.lR > cld(P.lR, P.nR - 1) || cc.lz < cld(P.lz, 0) ||
+           cc.lz > cld(P.lz, P.nz - 1);
+}
+
+// The outer PowerLaw expansion of a point outside the box, for ORD.
+template <int ORD>
+__device__ __forceinline__ void cylspline_outside(const CSK &P, double x, double y, double z,
+                                                  const CylCoords &cc, PointOut &o)
+{
+    constexpr bool G = ORD == ORD_GRAD || ORD == ORD_HESS;
+    constexpr bool H = ORD == ORD_HESS || ORD == ORD_DENS;   // density from the trace
+    cyl_outer<G || H, H, ORD == ORD_DENS>(P, x, y, z, cc.R, cc.iR, o);
+}
+
+template <int ORD>
+__device__ __forceinline__ void cylspline_inside(const CSK &P, double x, double y, double z,
+                                                 const CylCoords &cc, PointOut &o);
+
 // One point of a CylSpline potential.  ORD as the Multipole kernels.
 template <int ORD>
 __device__ __forceinline__ void cylspline_point(const CSK &P, double x, double y, double z,
                                                 PointOut &o)
 {
+    const CylCoords cc = cyl_coords(P, x, y, z);
+    if (cyl_outside(P, cc))
+        cylspline_outside<ORD>(P, x, y, z, cc, o);
+    else
+        cylspline_inside<ORD>(P, x, y, z, cc, o);
+}
+
+// The bicubic spline of a point inside the box.
+template <int ORD>
+__device__ __forceinline__ void cylspline_inside(const CSK &P, double x, double y, double z,
+                                                 const CylCoords &cc, PointOut &o)
+{
     constexpr bool G = ORD == ORD_GRAD || ORD == ORD_HESS;
     constexpr bool H = ORD == ORD_HESS || ORD == ORD_DENS;   // density from the trace
-    const CylCoords cc = cyl_coords(P, x, y, z);
     const double R = cc.R, lR = cc.lR, lz = cc.lz;
-    if (lR < cld(P.lR, 0) || lR > cld(P.lR, P.nR - 1) || lz < cld(P.lz, 0) ||
-        lz > cld(P.lz, P.nz - 1)) {
-        cyl_outer<G || H, H, ORD == ORD_DENS>(P, x, y, z, R, cc.iR, o);
-        return;
-    }
     const int iR = cell_search(lR, P.lR, P.nR), iz = cell_search(lz, P.lz, P.nz);
     const double lR0 = cld(P.lR, iR), lR1 = cld(P.lR, iR + 1);
     const double lz0 = cld(P.lz, iz), lz1 = cld(P.lz, iz + 1);

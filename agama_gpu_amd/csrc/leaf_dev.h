@@ -11,6 +11,30 @@
 
 namespace agx {
 
+// Adds one leaf's value (rho for ORD_DENS), gradient and Hessian with the
+// leaf's weights ampl, ampl sigma, ampl sigma^2 (rho: ampl sigma^2).
+template <int ORD>
+__device__ __forceinline__ void accumulate(double phi, double rho, const double eg[3],
+                                           const double eh[6], double ampl, double sg, double &P,
+                                           double g[3], double h[6])
+{
+    constexpr bool G = ORD == ORD_GRAD || ORD == ORD_HESS;
+    constexpr bool H = ORD == ORD_HESS;
+    if (ORD == ORD_DENS) {
+        P = fma(ampl * sg * sg, rho, P);
+    } else {
+        P = fma(ampl, phi, P);
+        if (G) {
+            const double ag = ampl * sg;
+            for (int k = 0; k < 3; ++k) g[k] = fma(ag, eg[k], g[k]);
+        }
+        if (H) {
+            const double ah = ampl * sg * sg;
+            for (int k = 0; k < 6; ++k) h[k] = fma(ah, eh[k], h[k]);
+        }
+    }
+}
+
 // EXP = false: the caller guarantees an analytic type (no expansion code is
 // compiled in, which keeps analytic-only kernels small in registers).
 template <int ORD, bool EXP = true, int STRIDE = 0, bool SIN = true>
@@ -48,19 +72,39 @@ __device__ __forceinline__ void add_leaf(int type, const double *p, const void *
     } else {
         eval_analytic<G, H, RHO>(type, p, x, y, z, e);
     }
-    if (RHO) {
-        P = fma(ampl * sg * sg, e.rho, P);
-    } else {
-        P = fma(ampl, e.phi, P);
-        if (G) {
-            const double ag = ampl * sg;
-            for (int k = 0; k < 3; ++k) g[k] = fma(ag, e.g[k], g[k]);
-        }
-        if (H) {
-            const double ah = ampl * sg * sg;
-            for (int k = 0; k < 6; ++k) h[k] = fma(ah, e.h[k], h[k]);
-        }
-    }
+    accumulate<ORD>(e.phi, e.rho, e.g, e.h, ampl, sg, P, g, h);
+}
+
+// A CylSpline leaf in two halves, for callers that defer the points outside
+// its grid box: cyl_leaf_inside adds the spline part and returns true, or
+// adds nothing and returns false for a point outside the box;
+// cyl_leaf_outside adds the outer expansion of such a point.
+template <int ORD>
+__device__ __forceinline__ bool cyl_leaf_inside(const agx_component &C, double x0, double y0,
+                                                double z0, double &P, double g[3], double h[6])
+{
+    const double sg = C.scale;
+    const double x = sg * (x0 - C.center[0]), y = sg * (y0 - C.center[1]), z = sg * (z0 - C.center[2]);
+    const CSK cs = load_desc<CSK>(C.table);
+    const CylCoords cc = cyl_coords(cs, x, y, z);
+    if (cyl_outside(cs, cc)) return false;
+    PointOut o{};
+    cylspline_inside<ORD>(cs, x, y, z, cc, o);
+    accumulate<ORD>(o.phi, o.phi, o.g, o.h, C.ampl, sg, P, g, h);
+    return true;
+}
+
+template <int ORD>
+__device__ __forceinline__ void cyl_leaf_outside(const agx_component &C, double x0, double y0,
+                                                 double z0, double &P, double g[3], double h[6])
+{
+    const double sg = C.scale;
+    const double x = sg * (x0 - C.center[0]), y = sg * (y0 - C.center[1]), z = sg * (z0 - C.center[2]);
+    const CSK cs = load_desc<CSK>(C.table);
+    const CylCoords cc = cyl_coords(cs, x, y, z);
+    PointOut o{};
+    cylspline_outside<ORD>(cs, x, y, z, cc, o);
+    accumulate<ORD>(o.phi, o.phi, o.g, o.h, C.ampl, sg, P, g, h);
 }
 
 }  // namespace agx

@@ -314,6 +314,8 @@ extern "C" int agx_set_option(int key, int64_t value)
         if (value < 0 || value > 4096) return AGX_E_SIZE;
         g_sort_grid = (int)value;
         return AGX_OK;
+    case AGX_OPT_CYL_PATH:
+        return cyl_set_path(value);
     default:
         return AGX_E_WHAT;
     }

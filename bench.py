@@ -80,25 +80,45 @@ def classify(desc: dict, x: np.ndarray) -> np.ndarray:
     return kinds
 
 
-def profiled_traffic(workload_file: str):
-    """HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs
-    (copy-calibrated) for this workload, read from its committed summary."""
+def profiled_value(workload_file: str, key: str):
+    """A figure recorded in a committed rocprof summary as ``key = value``:
+    HBM bytes per launch from the copy-calibrated FETCH_SIZE / WRITE_SIZE runs
+    (traffic_bytes_per_launch), or FP64 flops executed per launch from the
+    SQ_INSTS_VALU_*_F64 counters (fp64_flops_per_launch)."""
     path = os.path.join(ROOT, "profiles", workload_file)
     try:
         for ln in open(path):
-            if ln.startswith("traffic_bytes_per_launch"):
-                return int(ln.split("=", 1)[1])
+            if ln.startswith(key):
+                return int(float(ln.split("=", 1)[1]))
     except OSError:
         pass
     return None
 
 
-def cpu_threads() -> int:
+def cpu_share() -> dict:
+    """The host cores this process may use: its affinity mask, the cgroup CPU
+    quota (cpu.max, in cores) when one is set, and OMP_NUM_THREADS when set;
+    ``threads`` is the smallest of them (the CPU baseline's thread count)."""
     try:
-        n = len(os.sched_getaffinity(0))
+        aff = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
+        aff = os.cpu_count() or 1
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS")
+    omp = int(omp) if omp and omp.isdigit() else None
+    cands = [aff] + [math.ceil(quota)] * (quota is not None) + [omp] * (omp is not None)
+    return {"threads": max(1, min(cands)), "affinity_cores": aff, "cgroup_quota_cores": quota,
+            "omp_num_threads": omp, "nproc": os.cpu_count()}
+
+
+def cpu_threads() -> int:
+    return cpu_share()["threads"]
 
 
 def cpu_model() -> str:
@@ -153,6 +173,23 @@ def bench_batched(dev, reps: int = 5):
     ref = force.clone()
     wall_s, gpu_s = timed(lambda: ens.eval_separately(xyz, phi, force))
     agree = float((torch.max(torch.abs(force - ref)) / torch.max(torch.abs(ref))).item())
+    # the per-potential launches captured in one HIP graph and replayed: the
+    # same kernels without the host's launch gaps between them
+    side = torch.cuda.Stream(dev)
+    side.wait_stream(stream)
+    with torch.cuda.stream(side):
+        ens.eval_separately(xyz, phi, force)
+    stream.wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        ens.eval_separately(xyz, phi, force)
+    force.fill_(0.0)
+    torch.cuda.synchronize()
+    g.replay()
+    torch.cuda.synchronize()
+    agree_graph = float((torch.max(torch.abs(force - ref)) / torch.max(torch.abs(ref))).item())
+    _, gpu_g = timed(g.replay)
     paths = {}
     for (pth, rows) in ens.choices:
         k = f"{['lane', 'lds'][pth]}/{rows}"
@@ -161,7 +198,9 @@ def bench_batched(dev, reps: int = 5):
             "ms_batched": wall_b, "ms_batched_events": gpu_b,
             "ms_per_potential_launches": wall_s, "ms_per_potential_events": gpu_s,
             "speedup": wall_s / wall_b, "particles_per_s_batched": n / (wall_b * 1e-3),
-            "max_rel_diff_vs_per_potential": agree, "choices": paths,
+            "ms_per_potential_graph": gpu_g, "speedup_vs_graph": gpu_g / gpu_b,
+            "max_rel_diff_vs_per_potential": agree, "max_rel_diff_graph": agree_graph,
+            "choices": paths,
             "setup_s": t_tune - t_setup, "tune_s": t_ready - t_tune}
 
 
@@ -232,6 +271,35 @@ def bench_composite(dev, n: int = 1_000_000):
     return {"components": "Multipole lmax 8 + CylSpline 32x32 mmax 6 + NFW + Plummer",
             "n_particles": n, "kernel_ms_median": ms, "particles_per_s": n / (ms * 1e-3),
             "launches": 1}
+
+
+def bench_cylspline(dev, n: int = 1_000_000):
+    """The CylSpline disk + bar table of the Milky-Way-like composite alone
+    (Miyamoto-Nagai disk plus m = 2, 4, 6 bar, 32 x 32 asinh grid, 13
+    harmonics) at N=1M Hernquist points (seed 11): pot+force and evalDeriv,
+    with the algorithmic flop counts 150 + 150 n_harm and 250 + 220 n_harm
+    per point."""
+    import torch
+    from agama_gpu_amd import CylSplinePotentialGPU
+    from agama_gpu_amd.workloads import hernquist_points, mw_disk_bar_coefs
+    pot = CylSplinePotentialGPU(mw_disk_bar_coefs(), device=dev)
+    nh = int(pot._data["n_harm"])
+    xyz = torch.as_tensor(hernquist_points(n, 11), device=dev)
+    phi = torch.empty(n, dtype=torch.float64, device=dev)
+    force = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    deriv = torch.empty((n, 6), dtype=torch.float64, device=dev)
+    out = {"n_particles": n, "n_harm": nh, "grid": [int(pot._data["lR_grid"].size),
+                                                    int(pot._data["lz_grid"].size)]}
+    for name, fn, fl in (("pot_force", lambda: pot.eval_into(xyz, phi=phi, force=force),
+                          150 + 150 * nh),
+                         ("evalderiv", lambda: pot.eval_into(xyz, phi=phi, force=force,
+                                                             deriv=deriv), 250 + 220 * nh)):
+        ms = float(np.median(_event_times(fn)))
+        tf = fl * n / (ms * 1e-3) / 1e12
+        out[name] = {"kernel_ms_median": ms, "particles_per_s": n / (ms * 1e-3),
+                     "flops_per_point": fl, "achieved_tflops": tf,
+                     "frac_fp64_peak": tf / PEAK_FP64_TFLOPS}
+    return out
 
 
 def bench_high_order(dev, n: int = 10_000_000):
@@ -339,8 +407,8 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-ensemble and gradient legs")
-    ap.add_argument("--only", choices=["dense", "analytic", "composite", "high_order", "batched",
-                                       "grad", "leapfrog"],
+    ap.add_argument("--only", choices=["dense", "analytic", "composite", "cylspline", "high_order",
+                                       "batched", "grad", "leapfrog"],
                     help="run one secondary leg alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy three times (calibrates the "
@@ -365,10 +433,11 @@ def main():
     from agama_gpu_amd import MultipolePotentialGPU
     from agama_gpu_amd.workloads import halo_lmax8_coefs, hernquist_points
 
-    if args.only in ("composite", "high_order", "batched", "grad", "leapfrog"):
+    if args.only in ("composite", "cylspline", "high_order", "batched", "grad", "leapfrog"):
         dev = torch.device("cuda", 0)
-        leg = {"composite": bench_composite, "high_order": bench_high_order,
-               "batched": bench_batched, "grad": bench_grad, "leapfrog": bench_leapfrog}[args.only]
+        leg = {"composite": bench_composite, "cylspline": bench_cylspline,
+               "high_order": bench_high_order, "batched": bench_batched, "grad": bench_grad,
+               "leapfrog": bench_leapfrog}[args.only]
         print(json.dumps({args.only: leg(dev)}), flush=True)
         return
 
@@ -386,8 +455,9 @@ def main():
     force = torch.empty((N, 3), dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
     if args.only:
-        leg = {"dense": bench_dense, "analytic": bench_analytic}[args.only]
-        print(json.dumps({args.only: leg(dev, x_np, xyz, phi, force)}), flush=True)
+        res = (bench_dense(dev, x_np, xyz, phi, force) if args.only == "dense"
+               else bench_analytic(dev, xyz, phi, force))
+        print(json.dumps({args.only: res}), flush=True)
         return
 
     def step():
@@ -446,21 +516,28 @@ def main():
             orc.pot_force(x_np, th)
             ts.append(time.perf_counter() - c0)
         t_cpu = float(np.median(ts))
+        share = cpu_share()
         cpu = {"value": N / t_cpu, "unit": "particles/s", "cores": th, "kind": "port",
+               **{k: v for k, v in share.items() if k != "threads"},
                "sample": f"full N={N} workload, median of {args.cpu_reps} after 2 warmups; "
-                         f"oracle/mp_oracle.c -O3 OpenMP on {cpu_model()}"}
+                         f"oracle/mp_oracle.c -O3 OpenMP, {th} threads (the smallest of the "
+                         f"affinity mask, the cgroup quota and OMP_NUM_THREADS) on "
+                         f"{cpu_model()}"}
 
     extras = {}
     if not args.no_extras:
         extras["dense"] = bench_dense(dev, x_np, xyz, phi, force)
         extras["analytic"] = bench_analytic(dev, xyz, phi, force)
         extras["composite"] = bench_composite(dev)
+        extras["cylspline"] = bench_cylspline(dev)
         extras["high_order"] = bench_high_order(dev)
         extras["batched"] = bench_batched(dev)
         extras["grad"] = bench_grad(dev)
         extras["leapfrog"] = bench_leapfrog(dev)
 
-    traffic = profiled_traffic("multipole_halo_lmax8_pot_force_N10M.md")
+    prof = "multipole_halo_lmax8_pot_force_N10M.md"
+    traffic = profiled_value(prof, "traffic_bytes_per_launch")
+    f64 = profiled_value(prof, "fp64_flops_per_launch")
     line = {
         "metric": "FP64 particles/s (Multipole lmax=8, pot+force, N=10M); batched+grad vs plain",
         "value": value,
@@ -485,7 +562,13 @@ def main():
                      "algorithmic_bytes": 56 * N,
                      "kernel_ms": k_mean, "kernel_ms_median": k_median,
                      "hbm_gbs": (traffic if traffic else 56 * N) / (k_mean * 1e-3) / 1e9,
-                     "flops_per_launch": flops, "opcount": opinfo},
+                     "flops_per_launch": flops, "opcount": opinfo,
+                     # FP64 flops the kernel executes (rocprof SQ_INSTS_VALU_{FMA,MUL,ADD,
+                     # TRANS}_F64 x 64 lanes, FMA = 2) over the same launch time
+                     "fp64_executed_per_launch": f64,
+                     "fp64_executed_tflops": (f64 / (k_mean * 1e-3) / 1e12) if f64 else None,
+                     "fp64_executed_frac": (f64 / (k_mean * 1e-3) / 1e12 / PEAK_FP64_TFLOPS)
+                     if f64 else None},
         "cpu_baseline": cpu,
         **extras,
     }

@@ -415,6 +415,10 @@ int agx_leapfrog(const agx_component *d_leaves, int32_t n_leaves,
 #define AGX_OPT_SORT_GRID 9   /* workgroups of the interval / cell sort's count and
                                  scatter launches (1..4096; default 0: about one
                                  per 4096 points, 64 to 1024) */
+#define AGX_OPT_CYL_PATH 10   /* CylSpline evaluation: 0 (default) points outside
+                                 the grid box deferred to a workgroup queue and
+                                 evaluated in full wavefronts, 1 one point per
+                                 lane whichever branch it takes */
 int agx_set_option(int key, int64_t value);
 
 /* Library identification and error text.  agx_version: major * 10000 +

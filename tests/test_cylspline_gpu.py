@@ -35,22 +35,43 @@ def points(n, seed):
     return x
 
 
-def test_cylspline_parity(setup):
+@pytest.fixture(params=[0, 1], ids=["deferred", "per_lane"])
+def cyl_path(request):
+    """Both CylSpline kernels: points outside the grid deferred to full
+    wavefronts (default), and one point per lane."""
+    from agama_gpu_amd import _lib
+    _lib.set_option(_lib.AGX_OPT_CYL_PATH, request.param)
+    yield request.param
+    _lib.set_option(_lib.AGX_OPT_CYL_PATH, 0)
+
+
+@pytest.mark.parametrize("n", [20000, 1, 255, 257, 70001])
+def test_cylspline_parity(setup, cyl_path, n):
     gpu, orc = setup
-    x = points(20000, 0)
-    print("cylspline N=20000", flush=True)
+    x = points(max(n, 8), n)[:n]
+    print(f"cylspline N={n} path={cyl_path}", flush=True)
     p, F, D = [t.cpu().numpy() for t in gpu.evalDeriv(torch.as_tensor(x, device="cuda"))]
     P, g, H, outer = orc.eval(x)
-    assert 0.05 < outer.mean() < 0.95
+    if n >= 255:
+        assert 0.05 < outer.mean() < 0.95
     assert np.max(np.abs(p - P) / np.abs(P)) < 1e-12
     assert np.max(np.abs(F + g)) <= 1e-11 * np.abs(g).max()
     ins = ~outer
-    assert np.max(np.abs(D[ins] + H[ins])) <= 1e-10 * np.abs(H[ins]).max()
-    assert np.max(np.abs(D[outer] + H[outer])) <= 1e-6 * np.abs(H[outer]).max()
-    rho = gpu.density(torch.as_tensor(x, device="cuda")).cpu().numpy()
+    if ins.any():
+        assert np.max(np.abs(D[ins] + H[ins])) <= 1e-10 * np.abs(H[ins]).max()
+    if outer.any():
+        assert np.max(np.abs(D[outer] + H[outer])) <= 1e-6 * np.abs(H[outer]).max()
+    xt = torch.as_tensor(x, device="cuda")
+    rho = gpu.density(xt).cpu().numpy()
     tr = (H[:, 0] + H[:, 1] + H[:, 2]) / (4 * np.pi * 4.30091727067736e-06)
-    assert np.max(np.abs(rho[ins] - tr[ins])) <= 1e-9 * np.abs(tr[ins]).max()
+    if ins.any():
+        assert np.max(np.abs(rho[ins] - tr[ins])) <= 1e-9 * np.abs(tr[ins]).max()
     assert np.all(rho[outer] == 0.0)
+    # pot-only and pot+force kernels
+    p1 = gpu.potential(xt).cpu().numpy()
+    p2, F2 = [t.cpu().numpy() for t in gpu.eval(xt, pot=True, acc=True)]
+    assert np.max(np.abs(p1 - P) / np.abs(P)) < 1e-12 and np.max(np.abs(p2 - P) / np.abs(P)) < 1e-12
+    assert np.max(np.abs(F2 + g)) <= 1e-11 * np.abs(g).max()
 
 
 def test_cylspline_in_fused_composite(setup):
