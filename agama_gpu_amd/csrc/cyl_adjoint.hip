@@ -109,11 +109,12 @@ __device__ __forceinline__ void cyl_adjoint_inner(const CSK &P, double x, double
     const double lR0 = cld(P.lR, iR), lR1 = cld(P.lR, iR + 1);
     const double lz0 = cld(P.lz, iz), lz1 = cld(P.lz, iz + 1);
     const double hR = lR1 - lR0, hz = lz1 - lz0;
-    const double tR = (lR - lR0) / hR, tz = (lz - lz0) / hz;
+    const double ihR = rcp_nr(hR), ihz = rcp_nr(hz);
+    const double tR = (lR - lR0) * ihR, tz = (lz - lz0) * ihz;
     double A[4], dA[4], B[4], dB[4];
     herm_basis(tR, hR, A, dA);
     herm_basis(tz, hz, B, dB);
-    for (int k = 0; k < 4; ++k) { dA[k] /= hR; dB[k] /= hz; }   // d/dlR, d/dlz
+    for (int k = 0; k < 4; ++k) { dA[k] *= ihR; dB[k] *= ihz; }   // d/dlR, d/dlz
     double cph = 1.0, sph = 0.0;
     if (R > 1.0e-300) { cph = x / R; sph = y / R; }
     const double Rs2 = P.Rscale * P.Rscale;

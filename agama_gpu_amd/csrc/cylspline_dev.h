@@ -69,16 +69,18 @@ __device__ __forceinline__ int cell_search(double x, const double *g, int n)
 }
 
 // cubic Hermite on [xl, xh] from values fl, fh and slopes dl, dh
-// (the reference's evalCubic1D, :110-128)
+// (the reference's evalCubic1D, :110-128); h = xh - xl and ih = 1/h, the
+// reciprocal formed once per cell (an IEEE division here, two per call and
+// eleven per harmonic, was a quarter of the evalDeriv instruction stream)
 template <bool D1, bool D2>
-__device__ __forceinline__ void herm1(double t, double h, double fl, double fh, double dl, double dh,
-                                      double &f, double &df, double &d2f)
+__device__ __forceinline__ void herm1(double t, double h, double ih, double fl, double fh,
+                                      double dl, double dh, double &f, double &df, double &d2f)
 {
     const double T = 1.0 - t, tT = t * T, dif = fh - fl;
-    const double Q = 3.0 * (dl + dh) - 6.0 * dif / h;
+    const double Q = 3.0 * (dl + dh) - 6.0 * dif * ih;
     f = fl * T + fh * t + (dif * (t - T) + (dl * T - dh * t) * h) * tT;
     if (D1) df = dl * T + dh * t - Q * tT;
-    if (D2) d2f = (dh - dl + Q * (t - T)) / h;
+    if (D2) d2f = (dh - dl + Q * (t - T)) * ih;
 }
 
 // Constants of the outer multipole's normalised Legendre recurrences for
@@ -253,7 +255,8 @@ __device__ __forceinline__ void cylspline_inside(const CSK &P, double x, double 
     const double lR0 = cld(P.lR, iR), lR1 = cld(P.lR, iR + 1);
     const double lz0 = cld(P.lz, iz), lz1 = cld(P.lz, iz + 1);
     const double hR = lR1 - lR0, hz = lz1 - lz0;
-    const double tR = (lR - lR0) * rcp_nr(hR), tz = (lz - lz0) * rcp_nr(hz);
+    const double ihR = rcp_nr(hR), ihz = rcp_nr(hz);
+    const double tR = (lR - lR0) * ihR, tz = (lz - lz0) * ihz;
     double cph = 1.0, sph = 0.0;
     if (R > 0.0) { cph = x * cc.iR; sph = y * cc.iR; }
 
@@ -281,13 +284,13 @@ __device__ __forceinline__ void cylspline_inside(const CSK &P, double x, double 
         const double gu[4] = {b1.x, e1.x, b1.y, e1.y};
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            herm1<G || H, H>(tz, hz, fl[k], fu[k], gl[k], gu[k], Fz[k], dFz[k], d2Fz[k]);
+            herm1<G || H, H>(tz, hz, ihz, fl[k], fu[k], gl[k], gu[k], Fz[k], dFz[k], d2Fz[k]);
         // step 2: splines in lR
         double v, vR = 0, vRR = 0, vZ = 0, vRZ = 0, vZZ = 0, dummy;
-        herm1<G || H, H>(tR, hR, Fz[0], Fz[1], Fz[2], Fz[3], v, vR, vRR);
+        herm1<G || H, H>(tR, hR, ihR, Fz[0], Fz[1], Fz[2], Fz[3], v, vR, vRR);
         v += off;
-        if (G || H) herm1<H, false>(tR, hR, dFz[0], dFz[1], dFz[2], dFz[3], vZ, vRZ, dummy);
-        if (H) herm1<false, false>(tR, hR, d2Fz[0], d2Fz[1], d2Fz[2], d2Fz[3], vZZ, dummy, dummy);
+        if (G || H) herm1<H, false>(tR, hR, ihR, dFz[0], dFz[1], dFz[2], dFz[3], vZ, vRZ, dummy);
+        if (H) herm1<false, false>(tR, hR, ihR, d2Fz[0], d2Fz[1], d2Fz[2], d2Fz[3], vZZ, dummy, dummy);
         if (mv == 0) {
             F0 = v; F0R = vR; F0Z = vZ; F0RR = vRR; F0RZ = vRZ; F0ZZ = vZZ;
             if (!P.log_scaling) {
