@@ -38,7 +38,10 @@ struct CompositeArgs {
 // every lane takes one, adds the outer expansion and writes the outputs, and
 // the rest are taken at the end.  The outer expansion (lmax 8) then runs in
 // full wavefronts instead of in nearly every wavefront (almost every one of
-// them holds a point outside the box).  Every output is written once.
+// them holds a point outside the box).  The deferred leaf's grids are read
+// from a copy in LDS; a leaf whose grids do not fit there (nR + nz >
+// kGridLDS) is evaluated point by point instead.  Every output is written
+// once.
 template <int ORD, int STRIDE, bool SIN>
 __global__ __launch_bounds__(kBlock) void composite_kernel(CompositeArgs A,
                                                            const double *__restrict__ xyz,
@@ -51,9 +54,17 @@ __global__ __launch_bounds__(kBlock) void composite_kernel(CompositeArgs A,
     constexpr int NQ = 2 * kBlock;            // the queue never holds 2 kBlock points
     __shared__ int64_t q_idx[NQ];
     __shared__ double q_sum[(1 + (G ? 3 : 0) + (H ? 6 : 0)) * NQ];
+    __shared__ double grids[kGridLDS];         // the deferred leaf's lR and lz grids
     __shared__ int queued;
     const int t = threadIdx.x;
-    const int dc = ORD == ORD_DENS ? -1 : A.defer;   // the outer expansion has no density
+    // the deferred leaf (the outer expansion has no density), when its grids
+    // fit in LDS; otherwise it is evaluated point by point like other leaves
+    int dc = ORD == ORD_DENS ? -1 : A.defer;
+    if (dc >= 0) {
+        const CSK cs = load_desc<CSK>(A.c[dc].table);
+        if (cs.nR + cs.nz <= kGridLDS) stage_grids(cs, grids);   // (uniform branch)
+        else dc = -1;
+    }
     auto store = [&](int64_t i, double P, const double g[3], const double h[6]) {
         if (phi) __builtin_nontemporal_store(P, phi + i);    // streaming: written once
         if (G && force)
@@ -84,7 +95,7 @@ __global__ __launch_bounds__(kBlock) void composite_kernel(CompositeArgs A,
             for (int c = 0; c < A.n; ++c) {
                 const agx_component &C = A.c[c];
                 if (c == dc)
-                    later = !cyl_leaf_inside<ORD>(C, x0, y0, z0, P, g, h);
+                    later = !cyl_leaf_inside<ORD>(C, x0, y0, z0, P, g, h, grids);
                 else
                     add_leaf<ORD, true, STRIDE, SIN>(C.type, C.p, C.table, C.ampl, C.scale,
                                                      C.center, x0, y0, z0, P, g, h);

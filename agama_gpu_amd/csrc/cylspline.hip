@@ -38,7 +38,8 @@ __global__ __launch_bounds__(kBlock) void cyl_kernel(CSK P, const double *__rest
 // about as many instructions as the whole spline part) then runs in full
 // wavefronts; without the queue nearly every wavefront holds a point outside
 // the box and runs both branches.  The queue never holds 2 kBlock points:
-// before a block of points it holds fewer than kBlock.
+// before a block of points it holds fewer than kBlock.  The grids are read
+// from a copy in LDS (the launch requires nR + nz <= kGridLDS).
 template <int ORD>
 __global__ __launch_bounds__(kBlock) void cyl_deferred_kernel(CSK P, const double *__restrict__ xyz,
                                                               int64_t n, double *__restrict__ phi,
@@ -46,10 +47,11 @@ __global__ __launch_bounds__(kBlock) void cyl_deferred_kernel(CSK P, const doubl
                                                               double *__restrict__ deriv)
 {
     __shared__ int64_t queue[2 * kBlock];
+    __shared__ double grids[kGridLDS];         // lR then lz (nR + nz <= kGridLDS)
     __shared__ int queued;
     const int t = threadIdx.x;
     if (t == 0) queued = 0;
-    __syncthreads();
+    stage_grids(P, grids);
     auto outer = [&](int64_t j) {
         const double x = xyz[3 * j], y = xyz[3 * j + 1], z = xyz[3 * j + 2];
         const CylCoords cc = cyl_coords(P, x, y, z);
@@ -67,7 +69,7 @@ __global__ __launch_bounds__(kBlock) void cyl_deferred_kernel(CSK P, const doubl
                 queue[atomicAdd(&queued, 1)] = i;
             } else {
                 PointOut o{};
-                cylspline_inside<ORD>(P, x, y, z, cc, o);
+                cylspline_inside<ORD, true>(P, x, y, z, cc, o, grids, grids + P.nR);
                 cyl_store<ORD>(o, i, phi, force, deriv);
             }
         }
@@ -124,7 +126,7 @@ extern "C" int agx_cylspline_eval(const agx_cylspline *cs, const double *d_xyz, 
     double *force = (what & AGX_FORCE) ? d_force : nullptr;
     double *deriv = (what & AGX_DERIV) ? d_deriv : nullptr;
     dim3 grid(grid_for(n)), block(kBlock);
-    const bool defer = g_cyl_path == 0;
+    const bool defer = g_cyl_path == 0 && cs->nR + cs->nz <= kGridLDS;
     if (deriv)
         hipLaunchKernelGGL(defer ? cyl_deferred_kernel<ORD_HESS> : cyl_kernel<ORD_HESS>, grid, block,
                            0, s, P, d_xyz, n, phi, force, deriv);
@@ -145,7 +147,8 @@ extern "C" int agx_cylspline_density(const agx_cylspline *cs, const double *d_xy
     if (n == 0) return AGX_OK;
     if (!d_rho) return AGX_E_NULL;
     CSK P = make_cyl_args(cs);
-    hipLaunchKernelGGL(g_cyl_path == 0 ? cyl_deferred_kernel<ORD_DENS> : cyl_kernel<ORD_DENS>,
+    const bool defer = g_cyl_path == 0 && cs->nR + cs->nz <= kGridLDS;
+    hipLaunchKernelGGL(defer ? cyl_deferred_kernel<ORD_DENS> : cyl_kernel<ORD_DENS>,
                        dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, P, d_xyz, n, d_rho,
                        nullptr, nullptr);
     return hipGetLastError() == hipSuccess ? AGX_OK : AGX_E_LAUNCH;

@@ -57,15 +57,38 @@ __device__ __forceinline__ CylCoords cyl_coords(const CSK &P, double x, double y
     return c;
 }
 
+// A grid node: from the table (LDS = false: the constant address space, a
+// lane-divergent index is a vector load through the L1) or from a copy of the
+// grid in LDS (LDS = true: g points into a __shared__ array, and the binary
+// search's chain of dependent reads costs LDS latency instead of L1 latency).
+template <bool LDS>
+__device__ __forceinline__ double gnode(const double *g, int i)
+{
+    return LDS ? g[i] : cld(g, i);
+}
+
 // cell index i with grid[i] <= x < grid[i+1], clamped to [0, n-2]
+template <bool LDS = false>
 __device__ __forceinline__ int cell_search(double x, const double *g, int n)
 {
     int lo = 0, hi = n - 1;
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
-        if (cld(g, mid) > x) hi = mid; else lo = mid;
+        if (gnode<LDS>(g, mid) > x) hi = mid; else lo = mid;
     }
     return lo;
+}
+
+// Most grid nodes (lR and lz together) a kernel stages in LDS.
+constexpr int kGridLDS = 512;
+
+// Copy the lR then the lz grid into grids[0 .. nR + nz) (nR + nz <= kGridLDS);
+// every thread of the workgroup takes part, and a barrier follows.
+__device__ __forceinline__ void stage_grids(const CSK &P, double *grids)
+{
+    for (int k = threadIdx.x; k < P.nR + P.nz; k += blockDim.x)
+        grids[k] = k < P.nR ? cld(P.lR, k) : cld(P.lz, k - P.nR);
+    __syncthreads();
 }
 
 // cubic Hermite on [xl, xh] from values fl, fh and slopes dl, dh
@@ -227,9 +250,11 @@ __device__ __forceinline__ void cylspline_outside(const CSK &P, double x, double
     cyl_outer<G || H, H, ORD == ORD_DENS>(P, x, y, z, cc.R, cc.iR, o);
 }
 
-template <int ORD>
+template <int ORD, bool LDS = false>
 __device__ __forceinline__ void cylspline_inside(const CSK &P, double x, double y, double z,
-                                                 const CylCoords &cc, PointOut &o);
+                                                 const CylCoords &cc, PointOut &o,
+                                                 const double *gR = nullptr,
+                                                 const double *gz = nullptr);
 
 // One point of a CylSpline potential.  ORD as the Multipole kernels.
 template <int ORD>
@@ -243,17 +268,21 @@ __device__ __forceinline__ void cylspline_point(const CSK &P, double x, double y
         cylspline_inside<ORD>(P, x, y, z, cc, o);
 }
 
-// The bicubic spline of a point inside the box.
-template <int ORD>
+// The bicubic spline of a point inside the box.  LDS: the grids are read from
+// the LDS copies gR (nR nodes) and gz (nz nodes) that stage_grids made;
+// otherwise from the table.
+template <int ORD, bool LDS>
 __device__ __forceinline__ void cylspline_inside(const CSK &P, double x, double y, double z,
-                                                 const CylCoords &cc, PointOut &o)
+                                                 const CylCoords &cc, PointOut &o,
+                                                 const double *gR, const double *gz)
 {
     constexpr bool G = ORD == ORD_GRAD || ORD == ORD_HESS;
     constexpr bool H = ORD == ORD_HESS || ORD == ORD_DENS;   // density from the trace
+    if (!LDS) { gR = P.lR; gz = P.lz; }
     const double R = cc.R, lR = cc.lR, lz = cc.lz;
-    const int iR = cell_search(lR, P.lR, P.nR), iz = cell_search(lz, P.lz, P.nz);
-    const double lR0 = cld(P.lR, iR), lR1 = cld(P.lR, iR + 1);
-    const double lz0 = cld(P.lz, iz), lz1 = cld(P.lz, iz + 1);
+    const int iR = cell_search<LDS>(lR, gR, P.nR), iz = cell_search<LDS>(lz, gz, P.nz);
+    const double lR0 = gnode<LDS>(gR, iR), lR1 = gnode<LDS>(gR, iR + 1);
+    const double lz0 = gnode<LDS>(gz, iz), lz1 = gnode<LDS>(gz, iz + 1);
     const double hR = lR1 - lR0, hz = lz1 - lz0;
     const double ihR = rcp_nr(hR), ihz = rcp_nr(hz);
     const double tR = (lR - lR0) * ihR, tz = (lz - lz0) * ihz;

@@ -78,10 +78,12 @@ __device__ __forceinline__ void add_leaf(int type, const double *p, const void *
 // A CylSpline leaf in two halves, for callers that defer the points outside
 // its grid box: cyl_leaf_inside adds the spline part and returns true, or
 // adds nothing and returns false for a point outside the box;
-// cyl_leaf_outside adds the outer expansion of such a point.
+// cyl_leaf_outside adds the outer expansion of such a point.  grids: the
+// leaf's lR and lz grids as stage_grids copies them into LDS.
 template <int ORD>
 __device__ __forceinline__ bool cyl_leaf_inside(const agx_component &C, double x0, double y0,
-                                                double z0, double &P, double g[3], double h[6])
+                                                double z0, double &P, double g[3], double h[6],
+                                                const double *grids)
 {
     const double sg = C.scale;
     const double x = sg * (x0 - C.center[0]), y = sg * (y0 - C.center[1]), z = sg * (z0 - C.center[2]);
@@ -89,7 +91,7 @@ __device__ __forceinline__ bool cyl_leaf_inside(const agx_component &C, double x
     const CylCoords cc = cyl_coords(cs, x, y, z);
     if (cyl_outside(cs, cc)) return false;
     PointOut o{};
-    cylspline_inside<ORD>(cs, x, y, z, cc, o);
+    cylspline_inside<ORD, true>(cs, x, y, z, cc, o, grids, grids + cs.nR);
     accumulate<ORD>(o.phi, o.phi, o.g, o.h, C.ampl, sg, P, g, h);
     return true;
 }

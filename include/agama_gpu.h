@@ -417,7 +417,9 @@ int agx_leapfrog(const agx_component *d_leaves, int32_t n_leaves,
                                  per 4096 points, 64 to 1024) */
 #define AGX_OPT_CYL_PATH 10   /* CylSpline evaluation: 0 (default) points outside
                                  the grid box deferred to a workgroup queue and
-                                 evaluated in full wavefronts, 1 one point per
+                                 evaluated in full wavefronts, the grids read
+                                 from a copy in LDS (tables with nR + nz <= 512;
+                                 larger ones take path 1), 1 one point per
                                  lane whichever branch it takes */
 int agx_set_option(int key, int64_t value);
 

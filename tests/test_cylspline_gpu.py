@@ -99,3 +99,43 @@ def test_cylspline_in_fused_composite(setup):
     R = np.hypot(x[:, 0], x[:, 1])
     ok = ((R == 0) | (R > 1e-6 * np.linalg.norm(x, axis=1))) & ~outer
     assert np.max(np.abs(D[ok] - Dr[ok])) <= 1e-10 * np.abs(Dr[ok]).max()
+
+
+def test_cylspline_grid_beyond_lds():
+    """A table whose grids (200 + 320 nodes) exceed the 512 nodes the
+    deferred kernels copy into LDS: the standalone evaluation takes the
+    one-point-per-lane kernel and the composite evaluates the leaf point by
+    point; both equal the oracle (the exterior field of exact_fields on a
+    finer grid)."""
+    import math
+    from agama_gpu_amd import CompositePotentialGPU, CylSplinePotentialGPU, PotentialGPU
+    from agama_gpu_amd.coefs import CylSplineCoefs
+    from exact_fields import exterior_harmonics
+    from oracle import analytic_oracle as A
+    from oracle.cylspline_oracle import OracleCylSpline
+    R = np.concatenate([[0.0], 60.0 * np.sinh(np.linspace(0.05, 2.0, 199)) / math.sinh(2.0)])
+    R[-1] = 60.0
+    zp = 80.0 * np.sinh(np.linspace(0.02, 2.0, 160)) / math.sinh(2.0)
+    zp[-1] = 80.0
+    z = np.concatenate([-zp[::-1], zp])
+    RR, ZZ = np.meshgrid(R, z, indexing="ij")
+    s = np.maximum(1.0, 2.0 / np.hypot(RR, ZZ))
+    c = CylSplineCoefs(R, z, {m: np.asarray(t, float)
+                              for m, t in exterior_harmonics(RR * s, ZZ * s).items()}, 2)
+    gpu, orc = CylSplinePotentialGPU(c), OracleCylSpline(c)
+    x = points(20000, 7) * 0.15
+    P, g, H, outer = orc.eval(x)
+    assert 0.05 < outer.mean() < 0.95
+    print("cylspline 200x320 grid, N=20000", flush=True)
+    p, F, D = [t.cpu().numpy() for t in gpu.evalDeriv(torch.as_tensor(x, device="cuda"))]
+    assert np.max(np.abs(p - P) / np.abs(P)) < 1e-12
+    assert np.max(np.abs(F + g)) <= 1e-11 * np.abs(g).max()
+    ins = ~outer
+    assert np.max(np.abs(D[ins] + H[ins])) <= 1e-10 * np.abs(H[ins]).max()
+    comp = CompositePotentialGPU([gpu, PotentialGPU(type="Plummer", mass=1.0 / A.G,
+                                                    scaleRadius=0.5)])
+    pc, Fc = [t.cpu().numpy() for t in comp.eval(torch.as_tensor(x, device="cuda"), pot=True,
+                                                   acc=True)]
+    Pp, gp, _, _ = A.plummer(x, 1.0 / A.G, 0.5)
+    assert np.max(np.abs(pc - (P + Pp)) / np.abs(P + Pp)) < 1e-12
+    assert np.max(np.abs(Fc + g + gp)) <= 1e-11 * np.abs(g + gp).max()
