@@ -275,8 +275,8 @@ def bench_composite(dev, n: int = 1_000_000):
 
 def bench_cylspline(dev, n: int = 1_000_000):
     """The CylSpline disk + bar table of the Milky-Way-like composite alone
-    (Miyamoto-Nagai disk plus m = 2, 4, 6 bar, 32 x 32 asinh grid, 13
-    harmonics) at N=1M Hernquist points (seed 11): pot+force and evalDeriv,
+    (Miyamoto-Nagai disk plus m = 2, 4, 6 bar, 32 x 32 asinh grid mirrored
+    to 32 x 63, 7 harmonics) at N=1M Hernquist points (seed 11): pot+force and evalDeriv,
     with the algorithmic flop counts 150 + 150 n_harm and 250 + 220 n_harm
     per point."""
     import torch
