@@ -12,9 +12,9 @@
 // bin's last chunk is partial).  Each term's six coefficient sums are reduced
 // across the wavefront by lane swaps and added to a wavefront-private LDS
 // copy of that interval's table slab; the slab goes to memory (FP64 atomics)
-// only when the wavefront moves to another interval.  Wavefronts take runs of
-// consecutive chunks from a queue counter.  No reference counterpart: the
-// reference has no derivatives with respect to its tables.
+// only when the wavefront moves to another interval.  Wavefronts take
+// contiguous ranges or runs of chunks (for_each_sorted_chunk).  No reference
+// counterpart: the reference has no derivatives with respect to its tables.
 #include "multipole_dev.h"
 #include "sort_dev.h"
 
@@ -281,17 +281,15 @@ extern "C" int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xy
         sort_requeue(W, st);
     else
         sort_launch(MpBinKey{P, d_xyz}, n, nb, W, st);
-    // about two runs per resident wavefront (3 per SIMD)
-    const int run_len = sort_run_len(n, nb, 256LL * 12);
-    const int64_t chunks = (n + 63) / 64 + nb;
-    const int64_t wg_need = (chunks + run_len - 1) / run_len / (kAdjBlock / 64) + 1;
-    const unsigned grid = (unsigned)(wg_need < 1024 ? wg_need : 1024);
     const size_t slab_bytes = (size_t)mp->n_lm * 6 * sizeof(double) * (kAdjBlock / 64);
     const bool slab = slab_bytes <= (size_t)kAdjSlabLds;
     const bool s1 = mp->prog_flags & AGX_PROG_STRIDE1;
     auto kern = slab ? (s1 ? mp_adjoint_kernel<true, 1> : mp_adjoint_kernel<true, 0>)
                      : (s1 ? mp_adjoint_kernel<false, 1> : mp_adjoint_kernel<false, 0>);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kAdjBlock), slab ? slab_bytes : 0, st, P, d_xyz,
-                       d_gphi, d_gforce, d_gxyz, d_gpoly, d_gscal, W, run_len);
+    const size_t lds = slab ? slab_bytes : 0;
+    const SortedShape sh = sorted_shape(
+        n, nb, resident_workgroups((const void *)kern, kAdjBlock, lds, 256 * 3), kAdjBlock);
+    hipLaunchKernelGGL(kern, dim3(sh.grid), dim3(kAdjBlock), lds, st, P, d_xyz, d_gphi, d_gforce,
+                       d_gxyz, d_gpoly, d_gscal, W, sh.run_len);
     return hipGetLastError() == hipSuccess ? AGX_OK : AGX_E_LAUNCH;
 }

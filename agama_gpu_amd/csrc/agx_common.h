@@ -319,6 +319,12 @@ extern int g_grid_per_cu;
 int cyl_set_path(int64_t value);
 int cyl_path();
 
+// Workgroups of `kernel` (block threads, lds bytes of dynamic LDS each) that
+// are resident on the current device at once: the runtime's occupancy per CU
+// times the CU count.  Cached per device, kernel, block and LDS size;
+// `fallback` if the runtime cannot tell.  Defined in capi.hip.
+int64_t resident_workgroups(const void *kernel, int block, size_t lds, int64_t fallback);
+
 // Grid size for a grid-stride launch over n items: enough blocks to fill
 // 256 CUs several times over (oversubscribed, so that workgroups finishing
 // early free their slots for the remaining work), never more than needed.

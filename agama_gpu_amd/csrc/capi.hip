@@ -1,9 +1,43 @@
 // Library-level entry points of the C ABI (version, error text).
 #include "agx_common.h"
 
+#include <mutex>
+
 namespace agx {
 int g_grid_per_cu = 32;
+
+namespace {
+struct Resident {
+    int dev, block;
+    const void *kernel;
+    size_t lds;
+    int64_t wgs;
+};
+constexpr int kResidentSlots = 128;
+Resident g_resident[kResidentSlots];
+int g_resident_n = 0;
+std::mutex g_resident_mutex;
+}  // namespace
+
+int64_t resident_workgroups(const void *kernel, int block, size_t lds, int64_t fallback)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return fallback;
+    std::lock_guard<std::mutex> lock(g_resident_mutex);
+    for (int q = 0; q < g_resident_n; ++q) {
+        const Resident &e = g_resident[q];
+        if (e.dev == dev && e.kernel == kernel && e.block == block && e.lds == lds) return e.wgs;
+    }
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        per_cu < 1 || cus < 1)
+        return fallback;
+    const int64_t wgs = (int64_t)per_cu * cus;
+    if (g_resident_n < kResidentSlots) g_resident[g_resident_n++] = Resident{dev, block, kernel, lds, wgs};
+    return wgs;
 }
+}  // namespace agx
 
 // 1.1.0: agx_multipole.prog_flags, the program records' packed offsets and
 // gamma, the prog_flags argument of agx_multipole_batch_eval

@@ -355,12 +355,11 @@ extern "C" int agx_cylspline_adjoint(const agx_cylspline *cs, const double *d_xy
         if (work_bytes < sort_work_size(n, nb)) return AGX_E_WORKSPACE;
         const SortWork W = sort_work(d_work, n, nb);
         sort_launch(CylCellKey{P, d_xyz}, n, nb, W, st);
-        const int run_len = sort_run_len(n, nb, 256LL * 8);
-        const int64_t chunks = (n + 63) / 64 + nb;
-        const int64_t wg_need = (chunks + run_len - 1) / run_len / (kBlock / 64) + 1;
-        const unsigned grid = (unsigned)(wg_need < 1024 ? wg_need : 1024);
-        hipLaunchKernelGGL(cyl_adjoint_sorted_kernel, dim3(grid), dim3(kBlock), 0, st, P, d_xyz,
-                           d_gphi, d_gforce, d_gxyz, d_gnodes, d_gW, W, nb, run_len);
+        const SortedShape sh = sorted_shape(
+            n, nb, resident_workgroups((const void *)cyl_adjoint_sorted_kernel, kBlock, 0, 256 * 2),
+            kBlock);
+        hipLaunchKernelGGL(cyl_adjoint_sorted_kernel, dim3(sh.grid), dim3(kBlock), 0, st, P, d_xyz,
+                           d_gphi, d_gforce, d_gxyz, d_gnodes, d_gW, W, nb, sh.run_len);
     } else {
         const int64_t n_pad = (n + 63) & ~(int64_t)63;
         hipLaunchKernelGGL(cyl_adjoint_kernel, dim3(grid_for(n_pad)), dim3(kBlock), 0, st, P, d_xyz,

@@ -472,23 +472,30 @@ __device__ __forceinline__ void extrap(const MPK &P, const Where &w, double x, d
     radial_out<ORD>(x, y, z, w.ir, Pv, D, D2, o);
 }
 
-template <int ORD>
-// Outputs are written once and not read again by the kernel: streaming
-// (non-temporal) stores, which leave fewer dirty L2 lines to write back when
+// Outputs are written once and not read again by the kernel.  NT: streaming
+// (non-temporal) stores, for kernels whose wavefronts write runs of
+// consecutive particles; they leave fewer dirty L2 lines to write back when
 // the kernel ends (the halo table at N=10M: 4.0e10 -> 4.3e10 particles/s
-// over back-to-back launches).
+// over back-to-back launches).  Kernels that write particles in a scattered
+// order (the interval-sorted one) use ordinary stores, so that the partial
+// lines of neighbouring particles merge in L2 before they go to memory.
+template <int ORD, bool NT = true>
 __device__ __forceinline__ void store(const PointOut &o, int64_t i, double *phi, double *force,
                                       double *deriv)
 {
-    if (phi) __builtin_nontemporal_store(o.phi, phi + i);
+    auto put = [](double v, double *p) {
+        if (NT) __builtin_nontemporal_store(v, p);
+        else *p = v;
+    };
+    if (phi) put(o.phi, phi + i);
     if ((ORD == ORD_GRAD || ORD == ORD_HESS) && force) {
-        __builtin_nontemporal_store(-o.g[0], force + 3 * i);
-        __builtin_nontemporal_store(-o.g[1], force + 3 * i + 1);
-        __builtin_nontemporal_store(-o.g[2], force + 3 * i + 2);
+        put(-o.g[0], force + 3 * i);
+        put(-o.g[1], force + 3 * i + 1);
+        put(-o.g[2], force + 3 * i + 2);
     }
     if (ORD == ORD_HESS && deriv) {
 #pragma unroll
-        for (int p = 0; p < 6; ++p) __builtin_nontemporal_store(-o.h[p], deriv + 6 * i + p);
+        for (int p = 0; p < 6; ++p) put(-o.h[p], deriv + 6 * i + p);
     }
 }
 

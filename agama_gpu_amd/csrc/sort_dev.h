@@ -1,5 +1,5 @@
 // Counting sort of particle indices by a small integer key ("bin"), into a
-// caller-provided workspace, and the run queue that hands the sorted order to
+// caller-provided workspace, and the traversal that hands the sorted order to
 // whole wavefronts 64 slots at a time.  Used by the adjoint kernels: with the
 // particles of one bin (radial interval, spline cell) in every wavefront,
 // per-bin table gradients are summed across the wavefront before they go to
@@ -20,7 +20,7 @@ struct SortWork {
     int *hist;       // [nb] particles per bin
     int *off;        // [nb + 1] first slot of each bin
     int *cursor;     // [nb] next free slot of each bin while scattering
-    int *queue;      // [1] next run of chunks
+    int *queue;      // [1] next run of chunks (the dynamic traversal)
     uint16_t *bin;   // [n]
     int *idx;        // [n + 64 nb] particle of each slot
 };
@@ -164,29 +164,29 @@ inline void sort_launch(Key key, int64_t n, int nb, const SortWork &W, hipStream
                        nb, W);
 }
 
-// Runs of run_len consecutive 64-slot chunks, taken from the queue by whole
-// wavefronts until none is left.  For each chunk, body(bin, valid, i) is
-// called by all 64 lanes (padding lanes with valid = false, i = 0).  Must be
-// called by every lane of the wavefront.
+// The chunks of the sorted order, handed to whole wavefronts: for each chunk,
+// body(bin, valid, i) is called by all 64 lanes (padding lanes with valid =
+// false, i = 0).  Must be called by every lane of the wavefront.
+//
+// run_len > 0: runs of run_len consecutive chunks taken from the queue until
+// none is left, which balances the work to within a run.  run_len = 0:
+// static contiguous ranges, one per wavefront of the grid, with bin 0 (the
+// extrapolation bin: chunks of a fraction of the interior chunks' work) and
+// the interior chunks split separately, so every wavefront gets equal shares
+// of both.  The queue's counter serialises at its one L2 address (about 16 ns
+// per increment, measured), which costs more than the balance is worth when
+// each wavefront has only a few chunks; the launcher picks (sorted_shape).
+// Contiguous ranges also keep most of a wavefront's consecutive chunks in one
+// bin, so per-bin state (an LDS slab) is reloaded or flushed only where a bin
+// ends.
 template <class Body>
 __device__ __forceinline__ void for_each_sorted_chunk(const SortWork &W, int nb, int run_len,
                                                       Body body)
 {
     const int lane = threadIdx.x & 63;
     const int total = cldi(W.off, nb) >> 6;
-    for (;;) {
-        int run = 0;
-        if (lane == 0) run = atomicAdd(W.queue, 1);
-        run = __builtin_amdgcn_readfirstlane(run);
-        const int c0 = run * run_len;
-        if (c0 >= total) break;
-        const int c1 = min(c0 + run_len, total);
-        // bin of chunk c0: the last b with off[b] <= 64 c0 (off[0] = 0 <= 64 c0 < off[nb])
-        int b = 0, hi = nb;
-        while (hi - b > 1) {
-            const int mid = (b + hi) >> 1;
-            if (cldi(W.off, mid) <= c0 * 64) b = mid; else hi = mid;
-        }
+    // chunks [c0, c1) in order, from bin b0 on (the bin of c0 or an earlier one)
+    auto range = [&](int c0, int c1, int b) {
         for (int c = c0; c < c1; ++c) {
             while (cldi(W.off, b + 1) <= c * 64) ++b;
             const int sl = c * 64 + lane;
@@ -194,15 +194,58 @@ __device__ __forceinline__ void for_each_sorted_chunk(const SortWork &W, int nb,
             const int64_t i = valid ? (int64_t)W.idx[sl] : 0;
             body(b, valid, i);
         }
+    };
+    if (run_len == 0) {
+        const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+        const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) +
+                          __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        const int64_t ne = cldi(W.off, 1) >> 6, ni = total - ne;   // bin 0, interior
+        range((int)(w * ne / nw), (int)((w + 1) * ne / nw), 0);
+        const int i0 = (int)(ne + w * ni / nw), i1 = (int)(ne + (w + 1) * ni / nw);
+        if (i0 >= i1) return;
+        // bin of chunk i0: the last b with off[b] <= 64 i0
+        int b = 0, hi = nb;
+        while (hi - b > 1) {
+            const int mid = (b + hi) >> 1;
+            if (cldi(W.off, mid) <= i0 * 64) b = mid; else hi = mid;
+        }
+        range(i0, i1, b);
+        return;
+    }
+    for (;;) {
+        int run = 0;
+        if (lane == 0) run = atomicAdd(W.queue, 1);
+        run = __builtin_amdgcn_readfirstlane(run);
+        const int c0 = run * run_len;
+        if (c0 >= total) break;
+        int b = 0, hi = nb;
+        while (hi - b > 1) {
+            const int mid = (b + hi) >> 1;
+            if (cldi(W.off, mid) <= c0 * 64) b = mid; else hi = mid;
+        }
+        range(c0, min(c0 + run_len, total), b);
     }
 }
 
-// Chunks per run: about two runs per resident wavefront (waves), at least 1.
-inline int sort_run_len(int64_t n, int nb, int64_t waves)
+// Launch shape of a kernel that walks the sorted order with workgroups of
+// `block` threads, resident_wgs of which fit on the device at once: no more
+// workgroups than that (later ones would only start late), and with at least
+// 32 chunks per wavefront the queue with runs of a quarter of that share
+// (few increments per unit of work), otherwise static ranges (run_len 0).
+struct SortedShape {
+    unsigned grid;
+    int run_len;
+};
+
+inline SortedShape sorted_shape(int64_t n, int nb, int64_t resident_wgs, int block)
 {
+    const int64_t per_wg = block / 64;
     const int64_t chunks = (n + 63) / 64 + nb;
-    int64_t r = (chunks + 2 * waves - 1) / (2 * waves);
-    return (int)(r < 1 ? 1 : r);
+    int64_t wg = (chunks + per_wg - 1) / per_wg;
+    if (wg > resident_wgs) wg = resident_wgs;
+    if (wg < 1) wg = 1;
+    const int64_t share = chunks / (wg * per_wg);
+    return SortedShape{(unsigned)wg, share >= 32 ? (int)(share / 4) : 0};
 }
 
 }  // namespace agx
