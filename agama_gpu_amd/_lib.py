@@ -35,7 +35,7 @@ class agx_multipole(C.Structure):
         ("invPhi0", C.c_double),
         ("inner_s", C.c_double), ("inner_U", C.c_double), ("inner_W", C.c_double),
         ("outer_s", C.c_double), ("outer_U", C.c_double), ("outer_W", C.c_double),
-        ("law", C.c_void_p),
+        ("law", C.c_void_p), ("origin", C.c_void_p),
     ]
 
 
@@ -74,6 +74,7 @@ AGX_T_MULTIPOLE, AGX_T_CYLSPLINE = 16, 17
 
 AGX_PHI, AGX_FORCE, AGX_DERIV = 1, 2, 4
 AGX_WORK_SORTED = 1
+AGX_ADJ_CENTER_GRAD = 2
 AGX_PROG_STRIDE1 = 1
 AGX_PROG_COS_ONLY = 2
 AGX_PROG_STRIDE2 = 4

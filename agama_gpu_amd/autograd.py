@@ -87,11 +87,17 @@ def _quintic_operator(logr: np.ndarray):
 
 
 class _MultipoleEval(torch.autograd.Function):
-    """(xyz, poly, scal) -> (phi, force) with the adjoint kernel as backward."""
+    """(xyz, center, poly, scal) -> (phi, force) at xyz - center (center (3,)
+    or None) with the adjoint kernel as backward.  The kernels subtract the
+    centre themselves (agx_multipole.origin), and the centre's gradient,
+    minus the sum over particles of dL/dx, comes from the adjoint kernel too
+    (AGX_ADJ_CENTER_GRAD) instead of a separate column reduction."""
 
     @staticmethod
-    def forward(ctx, xyz, poly, scal, owner):
-        desc = owner._desc_for(poly, scal)
+    def forward(ctx, xyz, center, poly, scal, owner):
+        if center is not None:
+            center = center.contiguous()
+        desc = owner._desc_for(poly, scal, center)
         n = xyz.shape[0]
         phi = torch.empty(n, dtype=F64, device=xyz.device)
         force = torch.empty((n, 3), dtype=F64, device=xyz.device)
@@ -99,19 +105,25 @@ class _MultipoleEval(torch.autograd.Function):
         # the backward traverses the same order
         ctx.work = eval_table(desc, owner._d, xyz, phi, force, None,
                               _lib.AGX_PHI | _lib.AGX_FORCE, xyz.device)
-        ctx.save_for_backward(xyz, poly, scal)
+        ctx.save_for_backward(xyz, poly, scal, center)
         ctx.owner = owner
+        ctx.set_materialize_grads(False)      # an unused output reaches the kernel as NULL
         return phi, force
 
     @staticmethod
     def backward(ctx, gphi, gforce):
-        xyz, poly, scal = ctx.saved_tensors
+        if gphi is None and gforce is None:
+            return None, None, None, None, None
+        xyz, poly, scal, center = ctx.saved_tensors
         owner = ctx.owner
-        desc = owner._desc_for(poly, scal)
+        desc = owner._desc_for(poly, scal, center)
         n = xyz.shape[0]
+        want_c = ctx.needs_input_grad[1]
         gx = torch.empty_like(xyz) if ctx.needs_input_grad[0] else None
-        gpoly = torch.zeros_like(poly)
-        gscal = torch.zeros(5, dtype=F64, device=xyz.device)
+        # one buffer (one fill) for the table, law and centre gradients
+        gbuf = torch.zeros(poly.numel() + 8, dtype=F64, device=xyz.device)
+        gpoly = gbuf[:poly.numel()].view(poly.shape)
+        gscal = gbuf[poly.numel():] if want_c else gbuf[poly.numel():poly.numel() + 5]
         gphi = None if gphi is None else gphi.contiguous()
         gforce = None if gforce is None else gforce.contiguous()
         L = _lib.load()
@@ -120,13 +132,16 @@ class _MultipoleEval(torch.autograd.Function):
         ctx.work = None
         if work is None:
             work, flags = torch.empty(nwork, dtype=torch.uint8, device=xyz.device), 0
+        if want_c:
+            flags |= _lib.AGX_ADJ_CENTER_GRAD
         rc = L.agx_multipole_adjoint(
             C.byref(desc), xyz.data_ptr(), n, None if gphi is None else gphi.data_ptr(),
             None if gforce is None else gforce.data_ptr(), None if gx is None else gx.data_ptr(),
             gpoly.data_ptr(), gscal.data_ptr(), work.data_ptr(), nwork, flags,
             _lib.stream_handle(xyz.device))
         _lib.check(rc, "agx_multipole_adjoint")
-        return gx, gpoly, gscal, None
+        gc = gscal[5:] if want_c else None
+        return gx, gc, gpoly, gscal[:5], None
 
 
 class _MultipoleBuild(torch.autograd.Function):
@@ -159,8 +174,9 @@ class _MultipoleBuild(torch.autograd.Function):
         if glaw is None:
             glaw = torch.zeros(5, dtype=F64, device=dev)
         gpoly, glaw = gpoly.contiguous(), glaw.contiguous()
-        gphi = torch.zeros_like(phi_table)
-        gdphi = torch.zeros_like(dphi_table)
+        gtab = torch.zeros(2 * phi_table.numel(), dtype=F64, device=dev)   # one fill
+        gphi = gtab[:phi_table.numel()].view(phi_table.shape)
+        gdphi = gtab[phi_table.numel():].view(dphi_table.shape)
         work = torch.empty(1, dtype=F64, device=dev)
         rc = _lib.load().agx_multipole_build_adjoint(
             C.byref(src), gpoly.data_ptr(), glaw.data_ptr(), gphi.data_ptr(), gdphi.data_ptr(),
@@ -270,9 +286,12 @@ class FittableMultipole(torch.nn.Module):
         scal = torch.stack([invPhi0, U_in, W_in, U_out, W_out])
         return poly.contiguous(), scal
 
-    def _desc_for(self, poly: torch.Tensor, scal: torch.Tensor):
+    def _desc_for(self, poly: torch.Tensor, scal: torch.Tensor,
+                  center: Optional[torch.Tensor] = None):
         """Descriptor of the rebuilt table; the five law scalars stay in device
-        memory (agx_multipole.law), so no launch waits for a copy to the host."""
+        memory (agx_multipole.law), so no launch waits for a copy to the host,
+        and so does a centre (agx_multipole.origin: the kernels evaluate at
+        x - center without a shifted copy of the positions)."""
         d = self._d
         s = _lib.agx_multipole()
         s.poly_km = poly.data_ptr()
@@ -284,11 +303,13 @@ class FittableMultipole(torch.nn.Module):
         s.logr_min, s.dlogr, s.inv_dlogr = d["logr_min"], d["dlogr"], d["inv_dlogr"]
         s.inner_s, s.outer_s = d["inner_s"], d["outer_s"]
         s.law = scal.data_ptr()
+        s.origin = center.data_ptr() if center is not None else None
         return s
 
-    def forward(self, xyz: torch.Tensor):
+    def forward(self, xyz: torch.Tensor, center: Optional[torch.Tensor] = None):
+        """(phi, force) at xyz, or at xyz - center for a centre (3,)."""
         poly, scal = self.tables()
-        return _MultipoleEval.apply(xyz.contiguous(), poly, scal.contiguous(), self)
+        return _MultipoleEval.apply(xyz.contiguous(), center, poly, scal.contiguous(), self)
 
 
 def _spline_slope_matrix(x: np.ndarray, bc) -> np.ndarray:
@@ -536,4 +557,6 @@ class FittableShiftedMultipole(torch.nn.Module):
         self.center = center
 
     def forward(self, xyz: torch.Tensor, t: float = 0.0):
+        if isinstance(self.multipole, FittableMultipole):
+            return self.multipole(xyz, self.center(t))   # the shift inside the evaluation
         return self.multipole(_Shift.apply(xyz, self.center(t)))

@@ -15,6 +15,8 @@
 // only when the wavefront moves to another interval.  Wavefronts take
 // contiguous ranges or runs of chunks (for_each_sorted_chunk).  No reference
 // counterpart: the reference has no derivatives with respect to its tables.
+#include <type_traits>
+
 #include "multipole_dev.h"
 #include "sort_dev.h"
 
@@ -212,7 +214,9 @@ __device__ __forceinline__ void flush_slab(double *slab, double *g, int cnt, int
 // wavefronts per SIMD (at most 168 VGPRs: a few spill; measured faster than
 // two wavefronts without spills, 0.757 against 0.79 ms per gradient step).
 // STRIDE 1: the program recurs by 1 only (AGX_PROG_STRIDE1); 0: any program.
-template <bool SLAB, int STRIDE>
+// GSUM: also add minus the sum over particles of dL/dx to gscal[5..7] (the
+// gradient with respect to a centre subtracted from every position).
+template <bool SLAB, int STRIDE, bool GSUM>
 __global__ __launch_bounds__(kAdjBlock, 3) void mp_adjoint_kernel(
     MPK P, const double *__restrict__ xyz, const double *__restrict__ gphi,
     const double *__restrict__ gforce, double *__restrict__ gxyz, double *__restrict__ gpoly,
@@ -226,6 +230,11 @@ __global__ __launch_bounds__(kAdjBlock, 3) void mp_adjoint_kernel(
     if (SLAB)
         for (int q = lane; q < slab_n; q += 64) slab[q] = 0.0;
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};   // invPhi0, inner U, W, outer U, W
+    // GSUM: the wavefront's sum of dL/dx, in LDS (kept in registers across
+    // the evaluation it would cost spills)
+    __shared__ double gsum_lds[kAdjBlock / 64][4];
+    double *gsum = gsum_lds[threadIdx.x >> 6];
+    if (GSUM && lane < 3) gsum[lane] = 0.0;
     int cur = -1;                                // interval + 1 held by the slab
     for_each_sorted_chunk(W, P.n_int + 1, run_len, [&](int b, bool valid, int64_t i) {
         if (SLAB && b != cur) {
@@ -234,7 +243,7 @@ __global__ __launch_bounds__(kAdjBlock, 3) void mp_adjoint_kernel(
         }
         double x = 1.0, y = 0.0, z = 0.0, wphi = 0.0, wf[3] = {0.0, 0.0, 0.0};
         if (valid) {
-            x = xyz[3 * i]; y = xyz[3 * i + 1]; z = xyz[3 * i + 2];
+            load_pos(P, xyz, i, x, y, z);
             if (gphi) wphi = gphi[i];
             if (gforce) { wf[0] = gforce[3 * i]; wf[1] = gforce[3 * i + 1]; wf[2] = gforce[3 * i + 2]; }
         }
@@ -246,12 +255,23 @@ __global__ __launch_bounds__(kAdjBlock, 3) void mp_adjoint_kernel(
         else
             adjoint_extrap(P, valid, x, y, z, wphi, wf, gx, acc + 1);
         if (valid && gxyz) { gxyz[3 * i] = gx[0]; gxyz[3 * i + 1] = gx[1]; gxyz[3 * i + 2] = gx[2]; }
+        if (GSUM) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const double v = wave_sum(valid ? gx[q] : 0.0);
+                if (lane == 0) gsum[q] += v;
+            }
+        }
     });
     if (SLAB && cur > 0) flush_slab(slab, gpoly + (int64_t)(cur - 1) * slab_n, slab_n, lane);
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
         const double v = wave_sum(acc[q]);
         if (lane == 0 && v != 0.0) atomicAdd(gscal + q, v);
+    }
+    if (GSUM) {
+        __builtin_amdgcn_wave_barrier();
+        if (lane < 3 && gsum[lane] != 0.0) atomicAdd(gscal + 5 + lane, -gsum[lane]);
     }
 }
 
@@ -270,6 +290,7 @@ extern "C" int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xy
     if (mp->n_int < 1 || mp->n_int + 1 > kMaxBins || mp->n_lm < 1) return AGX_E_SIZE;
     if (mp->lmax > 32) return AGX_E_LMAX;
     if (mp->prog_flags & ~kProgFlags) return AGX_E_WHAT;
+    if (flags & ~(AGX_WORK_SORTED | AGX_ADJ_CENTER_GRAD)) return AGX_E_WHAT;
     if (n == 0) return AGX_OK;
     if (work_bytes < sort_work_size(n, kMaxBins)) return AGX_E_WORKSPACE;
     if (!d_xyz || (!d_gphi && !d_gforce) || !d_work) return AGX_E_NULL;
@@ -277,18 +298,22 @@ extern "C" int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xy
     const SortWork W = sort_work(d_work, n, kMaxBins);
     const int nb = mp->n_int + 1;
     hipStream_t st = (hipStream_t)stream;
-    if (flags & AGX_WORK_SORTED)
-        sort_requeue(W, st);
-    else
-        sort_launch(MpBinKey{P, d_xyz}, n, nb, W, st);
+    if (!(flags & AGX_WORK_SORTED)) sort_launch(MpBinKey{P, d_xyz}, n, nb, W, st);
     const size_t slab_bytes = (size_t)mp->n_lm * 6 * sizeof(double) * (kAdjBlock / 64);
     const bool slab = slab_bytes <= (size_t)kAdjSlabLds;
     const bool s1 = mp->prog_flags & AGX_PROG_STRIDE1;
-    auto kern = slab ? (s1 ? mp_adjoint_kernel<true, 1> : mp_adjoint_kernel<true, 0>)
-                     : (s1 ? mp_adjoint_kernel<false, 1> : mp_adjoint_kernel<false, 0>);
+    const bool gs = flags & AGX_ADJ_CENTER_GRAD;
+    auto pick = [&](auto gsum) {
+        constexpr bool G = decltype(gsum)::value;
+        return slab ? (s1 ? mp_adjoint_kernel<true, 1, G> : mp_adjoint_kernel<true, 0, G>)
+                    : (s1 ? mp_adjoint_kernel<false, 1, G> : mp_adjoint_kernel<false, 0, G>);
+    };
+    auto kern = gs ? pick(std::true_type{}) : pick(std::false_type{});
     const size_t lds = slab ? slab_bytes : 0;
     const SortedShape sh = sorted_shape(
         n, nb, resident_workgroups((const void *)kern, kAdjBlock, lds, 256 * 3), kAdjBlock);
+    // a reused sort has had its queue consumed; only the dynamic traversal reads it
+    if ((flags & AGX_WORK_SORTED) && sh.run_len > 0) sort_requeue(W, st);
     hipLaunchKernelGGL(kern, dim3(sh.grid), dim3(kAdjBlock), lds, st, P, d_xyz, d_gphi, d_gforce,
                        d_gxyz, d_gpoly, d_gscal, W, sh.run_len);
     return hipGetLastError() == hipSuccess ? AGX_OK : AGX_E_LAUNCH;

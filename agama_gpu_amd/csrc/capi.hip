@@ -44,7 +44,8 @@ int64_t resident_workgroups(const void *kernel, int block, size_t lds, int64_t f
 // 1.2.0: agx_component.prog_flags (formerly padding) selects the composite
 // kernel's Multipole variant, so callers built against 1.1 must zero it or set
 // it from the table's program; NaN positions give NaN results
-extern "C" int agx_version(void) { return 10200; }
+// 1.3.0: agx_multipole.origin (new last field), AGX_ADJ_CENTER_GRAD
+extern "C" int agx_version(void) { return 10300; }
 
 extern "C" const char *agx_error_string(int code)
 {

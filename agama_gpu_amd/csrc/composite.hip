@@ -315,6 +315,7 @@ extern "C" int agx_multipole_pack(const agx_multipole *mp, void *dst, int64_t si
 {
     if (!mp || !dst) return AGX_E_NULL;
     if (size < (int64_t)sizeof(MPK)) return AGX_E_WORKSPACE;
+    if (mp->origin) return AGX_E_WHAT;   // a leaf's centre is agx_component.center
     MPK P = make_args(mp);
     __builtin_memcpy(dst, &P, sizeof(MPK));
     return AGX_OK;

@@ -115,11 +115,48 @@ __global__ __launch_bounds__(kBuildThreads) void mp_build_kernel(BuildArgs A, do
     }
 }
 
+// The law scalars' dependence on the (0,0) column's end nodes, given
+// dL/dinvPhi0 (ginv): one thread, with atomics (the term kernels add to the
+// same entries concurrently).
+__device__ void law_adjoint(const BuildArgs &A, const double *__restrict__ glaw,
+                            double *__restrict__ gphi, double *__restrict__ gdphi, double ginv)
+{
+    const int nR = A.nR, col0 = A.cols[0];
+    const Law L = law_of(A, A.phi[col0], A.dphi[col0], A.phi[(nR - 1) * A.n_cols + col0],
+                         A.dphi[(nR - 1) * A.n_cols + col0]);
+    const double ra = A.r[0], rb = A.r[nR - 1];
+    double gWin = glaw[2];
+    double gp0a = 0.0, gd0a = 0.0;
+    if (A.log_scaling) {
+        if (A.inner_mode == 1) {
+            gp0a += gWin;
+            gd0a += glaw[1] * ra;
+        } else if (A.inner_mode == 2) {
+            if (!A.inner_reset) gWin -= ginv / (L.Win * L.Win);
+            gp0a += gWin;
+            gd0a += (glaw[1] - gWin) * 0.5 * ra;
+        } else {
+            gp0a += gWin;
+        }
+    } else {
+        gp0a += gWin;
+    }
+    atomicAdd(&gphi[col0], gp0a);
+    if (gd0a != 0.0) atomicAdd(&gdphi[col0], gd0a);
+    const int64_t e = (int64_t)(nR - 1) * A.n_cols + col0;
+    if (A.outer_mode == 1) {
+        atomicAdd(&gphi[e], glaw[3] + glaw[4]);
+        atomicAdd(&gdphi[e], glaw[3] * rb);
+    } else {
+        atomicAdd(&gphi[e], glaw[4]);
+    }
+}
+
 // Reverse of mp_build_kernel for term c: adds dL/dphi, dL/dphi' of column
 // cols[c] (plain adds for c > 0: one writer) and of the (0,0) column
 // (atomics, term 0's own included: every term's log-scaling depends on it).
 // Term 0 also writes work[0] = dL/dinvPhi0 (its own uses plus the law
-// output's).
+// output's) and adds the law scalars' gradients (law_adjoint).
 __global__ __launch_bounds__(kBuildThreads) void mp_build_adjoint_kernel(
     BuildArgs A, const double *__restrict__ gpoly, const double *__restrict__ glaw,
     double *__restrict__ gphi, double *__restrict__ gdphi, double *__restrict__ work)
@@ -196,45 +233,10 @@ __global__ __launch_bounds__(kBuildThreads) void mp_build_adjoint_kernel(
             if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
             __syncthreads();
         }
-        if (threadIdx.x == 0) work[0] = red[0] + glaw[0];
-    }
-}
-
-// The law scalars' dependence on the (0,0) column's end nodes (one thread,
-// after mp_build_adjoint_kernel).
-__global__ void mp_build_law_adjoint_kernel(BuildArgs A, const double *__restrict__ glaw,
-                                            double *__restrict__ gphi, double *__restrict__ gdphi,
-                                            const double *__restrict__ work)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const int nR = A.nR, col0 = A.cols[0];
-    const Law L = law_of(A, A.phi[col0], A.dphi[col0], A.phi[(nR - 1) * A.n_cols + col0],
-                         A.dphi[(nR - 1) * A.n_cols + col0]);
-    const double ra = A.r[0], rb = A.r[nR - 1];
-    double gWin = glaw[2];
-    double gp0a = 0.0, gd0a = 0.0;
-    if (A.log_scaling) {
-        if (A.inner_mode == 1) {
-            gp0a += gWin;
-            gd0a += glaw[1] * ra;
-        } else if (A.inner_mode == 2) {
-            if (!A.inner_reset) gWin -= work[0] / (L.Win * L.Win);
-            gp0a += gWin;
-            gd0a += (glaw[1] - gWin) * 0.5 * ra;
-        } else {
-            gp0a += gWin;
+        if (threadIdx.x == 0) {
+            work[0] = red[0] + glaw[0];
+            law_adjoint(A, glaw, gphi, gdphi, red[0] + glaw[0]);
         }
-    } else {
-        gp0a += gWin;
-    }
-    gphi[col0] += gp0a;
-    gdphi[col0] += gd0a;
-    const int64_t e = (int64_t)(nR - 1) * A.n_cols + col0;
-    if (A.outer_mode == 1) {
-        gphi[e] += glaw[3] + glaw[4];
-        gdphi[e] += glaw[3] * rb;
-    } else {
-        gphi[e] += glaw[4];
     }
 }
 
@@ -279,7 +281,5 @@ extern "C" int agx_multipole_build_adjoint(const agx_multipole_source *src, cons
     hipStream_t st = (hipStream_t)stream;
     hipLaunchKernelGGL(mp_build_adjoint_kernel, dim3(A.n_lm), dim3(kBuildThreads), 0, st, A, d_gpoly,
                        d_glaw, d_gphi, d_gdphi, d_work);
-    hipLaunchKernelGGL(mp_build_law_adjoint_kernel, dim3(1), dim3(64), 0, st, A, d_glaw, d_gphi,
-                       d_gdphi, d_work);
     return hipGetLastError() == hipSuccess ? AGX_OK : AGX_E_LAUNCH;
 }

@@ -14,6 +14,7 @@ __global__ __launch_bounds__(kBlock) void mp_lane_kernel(MPK P, const double *__
 {
     P = with_law(P);
     for_each_point<kBlock>(xyz, n, [&](int64_t i, double x, double y, double z) {
+        x -= P.ox; y -= P.oy; z -= P.oz;
         PointOut o{};
         Where w = classify(P, x, y, z);
         if (w.kind == 2)
@@ -33,7 +34,8 @@ __global__ __launch_bounds__(kBinThreads) void mp_binned_kernel(
     P = with_law(P);
     binned_tiles(P, xyz, n, tile, [&](int b, bool valid, int64_t i) {
         if (!valid) return;
-        const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+        double x, y, z;
+        load_pos(P, xyz, i, x, y, z);
         PointOut o{};
         if (b > 0) {
             const int k = b - 1;
@@ -77,7 +79,7 @@ __global__ __launch_bounds__(BLOCK) void mp_lds_kernel(MPK P, const double *__re
     __syncthreads();
     const int slab_len = P.n_lm * 6;
     for (; i < n; i += stride) {
-        const double x = xn, y = yn, z = zn;
+        const double x = xn - P.ox, y = yn - P.oy, z = zn - P.oz;
         const int64_t inx = i + stride;
         if (inx < n) { xn = xyz[3 * inx]; yn = xyz[3 * inx + 1]; zn = xyz[3 * inx + 2]; }
         PointOut o{};
@@ -125,7 +127,8 @@ __global__ __launch_bounds__(kBlock) void mp_sorted_kernel(MPK P, const double *
             staged = b - 1;
         }
         if (!valid) return;
-        const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+        double x, y, z;
+        load_pos(P, xyz, i, x, y, z);
         PointOut o{};
         if (b > 0) {
             const int k = b - 1;

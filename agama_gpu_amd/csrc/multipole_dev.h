@@ -39,10 +39,13 @@ struct MPK {
     double inv_rmin, inv_rmax;       // 1 / r at the first and last node
     double rmax;                     // r at the last node
     const double *law;               // NULL or device [invPhi0, inner U, W, outer U, W]
+    const double *origin;            // NULL or device [x0, y0, z0]: evaluated at x - origin
+    double ox, oy, oz;               // the origin (0 without one), set by with_law
 };
 
 // The table with its five law scalars taken from device memory when it
-// carries them there (agx_multipole.law); called once per kernel or descriptor.
+// carries them there (agx_multipole.law), and its origin; called once per
+// kernel or descriptor.
 __device__ __forceinline__ MPK with_law(MPK P)
 {
     if (P.law) {
@@ -50,7 +53,19 @@ __device__ __forceinline__ MPK with_law(MPK P)
         P.inner_U = cld(P.law, 1); P.inner_W = cld(P.law, 2);
         P.outer_U = cld(P.law, 3); P.outer_W = cld(P.law, 4);
     }
+    P.ox = P.oy = P.oz = 0.0;
+    if (P.origin) { P.ox = cld(P.origin, 0); P.oy = cld(P.origin, 1); P.oz = cld(P.origin, 2); }
     return P;
+}
+
+// Position i of an AoS (n, 3) array relative to the table's origin (x - 0 is
+// x exactly, so a table without one sees the positions unchanged).
+__device__ __forceinline__ void load_pos(const MPK &P, const double *xyz, int64_t i, double &x,
+                                         double &y, double &z)
+{
+    x = xyz[3 * i] - P.ox;
+    y = xyz[3 * i + 1] - P.oy;
+    z = xyz[3 * i + 2] - P.oz;
 }
 
 
@@ -543,7 +558,9 @@ __device__ __forceinline__ void binned_tile(const MPK &P, const double *__restri
     // phase 1: classify and histogram
     for (int j = tid; j < cnt_here; j += kBinThreads) {
         const int64_t i = base + j;
-        Where w = classify(P, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+        double x, y, z;
+        load_pos(P, xyz, i, x, y, z);
+        Where w = classify(P, x, y, z);
         const int b = (w.kind == 2) ? w.k + 1 : 0;
         L.pbin[j] = (uint8_t)b;
         atomicAdd(&L.cnt[b], 1);
@@ -595,11 +612,14 @@ __device__ __forceinline__ void binned_tiles(const MPK &P, const double *__restr
 // Sort key of the interval-sorted traversals (sort_dev.h): 0 outside the grid
 // (origin, inner and outer laws), k + 1 in interval k.
 struct MpBinKey {
-    MPK P;
+    MPK P;                           // as built on the host (make_args)
     const double *xyz;
     __device__ int operator()(int64_t i) const
     {
-        const Where w = classify(P, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+        const MPK Q = with_law(P);   // the origin (uniform loads, hoisted out of the loop)
+        double x, y, z;
+        load_pos(Q, xyz, i, x, y, z);
+        const Where w = classify(Q, x, y, z);
         return (w.kind == 2) ? w.k + 1 : 0;
     }
 };
@@ -628,6 +648,8 @@ inline MPK make_args(const agx_multipole *mp)
     P.inv_rmax = std::exp(-(mp->logr_min + (double)mp->n_int * mp->dlogr));
     P.rmax = std::exp(mp->logr_min + (double)mp->n_int * mp->dlogr);
     P.law = mp->law;
+    P.origin = mp->origin;
+    P.ox = P.oy = P.oz = 0.0;
     return P;
 }
 

@@ -13,7 +13,10 @@
 
 namespace agx {
 
-constexpr int kSortBlock = 256;
+// Threads per workgroup of the count and scatter kernels: large workgroups,
+// so that the grid (held small: see g_sort_grid) still holds several
+// wavefronts per SIMD to cover the latency of the position reads.
+constexpr int kSortBlock = 1024;
 constexpr int kSortMaxBins = 8192;
 
 struct SortWork {

@@ -102,6 +102,14 @@ typedef struct agx_multipole {
      * caller whose table is built on the device (torch.autograd) launch
      * without copying them to the host first. */
     const double *law;
+    /* NULL, or device memory holding a centre [x0, y0, z0] (1.3.0): the
+     * agx_multipole_* evaluation, density, sorted, no-allocation and adjoint
+     * entry points then evaluate at x - centre (the adjoint's dL/dx is still
+     * with respect to x).  A potential shifted along a fitted trajectory
+     * (torch.autograd) so needs no shifted copy of the positions.
+     * agx_multipole_pack rejects it (AGX_E_WHAT): a composite leaf's centre
+     * is agx_component.center. */
+    const double *origin;
 } agx_multipole;
 
 /* Potential / force / force derivatives at N points.
@@ -243,7 +251,11 @@ int agx_multipole_pack(const agx_multipole *mp, void *dst, int64_t size);
  * sorted by radial interval there); it is overwritten and needs no clearing.
  * flags: AGX_WORK_SORTED if d_work already holds the sort of these d_xyz
  * for this radial grid (left by agx_multipole_eval_sorted or an earlier
- * adjoint call with the same points and grid): the sort is skipped.
+ * adjoint call with the same points and grid): the sort is skipped;
+ * AGX_ADJ_CENTER_GRAD: also ADDS dL/dc into d_gscal[5..7] (which then has 8
+ * doubles) for positions that are x - c for a centre c: minus the sum over
+ * particles of dL/dx (d_gxyz may still be NULL); other bits give
+ * AGX_E_WHAT.
  * Table gradients are summed per wavefront and interval before FP64 atomic
  * adds, so their last bits depend on the order of the atomics.  Needs
  * n_int <= 127 and N < 2^31 - 8192.  No reference counterpart (the
@@ -254,6 +266,7 @@ int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xyz, int64_t 
                           double *d_gpoly, double *d_gscal, void *d_work, int64_t work_bytes,
                           uint32_t flags, void *stream);
 #define AGX_WORK_SORTED 1u
+#define AGX_ADJ_CENTER_GRAD 2u
 /* Device scratch (bytes) of the interval-sorted entry points for N points. */
 int64_t agx_multipole_work_size(int64_t n);
 
@@ -424,10 +437,12 @@ int agx_leapfrog(const agx_component *d_leaves, int32_t n_leaves,
 int agx_set_option(int key, int64_t value);
 
 /* Library identification and error text.  agx_version: major * 10000 +
- * minor * 100 + patch (10200: this header).  1.2.0: agx_component.prog_flags
+ * minor * 100 + patch (10300: this header).  1.2.0: agx_component.prog_flags
  * (formerly padding) is read; a position with a NaN coordinate gives NaN
  * outputs, and one whose r^2 overflows (|x| above about 1.3e154) gives the
- * outer-law limit 0 for Multipole tables. */
+ * outer-law limit 0 for Multipole tables.  1.3.0: agx_multipole.origin (a new
+ * last field: callers built against 1.2 pass a shorter struct and must be
+ * rebuilt) and agx_multipole_adjoint's AGX_ADJ_CENTER_GRAD flag. */
 int agx_version(void);
 const char *agx_error_string(int code);
 

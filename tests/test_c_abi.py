@@ -35,7 +35,7 @@ def test_library_exports_all_declared_symbols():
 def test_version_and_error_strings():
     from agama_gpu_amd import _lib
     L = _lib.load()
-    assert L.agx_version() == 10200
+    assert L.agx_version() == 10300
     assert b"NULL" in L.agx_error_string(-1)
     assert L.agx_error_string(12345) == b"unknown error code"
 
@@ -94,7 +94,7 @@ def test_descriptor_layout_matches_header():
     names = re.findall(r"\b([A-Za-z_0-9]+)\s*[,;]", body)
     assert names == [f[0] for f in _lib.agx_multipole._fields_]
     assert _lib.agx_multipole.logr_min.offset == 56
-    assert C.sizeof(_lib.agx_multipole) == 56 + 10 * 8 + 8
+    assert C.sizeof(_lib.agx_multipole) == 56 + 10 * 8 + 2 * 8
 
 
 def test_adjoint_workspace_validation_without_gpu():
@@ -112,6 +112,9 @@ def test_adjoint_workspace_validation_without_gpu():
     assert L.agx_multipole_adjoint(*args, None, need, 0, None) == -1
     assert L.agx_multipole_adjoint(*args, 8, need - 1, 0, None) == -7
     assert L.agx_multipole_adjoint(*args, 8, need, 0, None) == -1
+    # flags: AGX_ADJ_CENTER_GRAD is known (the NULL positions then fail), other bits are not
+    assert L.agx_multipole_adjoint(*args, 8, need, _lib.AGX_ADJ_CENTER_GRAD, None) == -1
+    assert L.agx_multipole_adjoint(*args, 8, need, 4, None) == -5      # AGX_E_WHAT
     d.n_int = 128                       # n_int + 1 bins > 128
     assert L.agx_multipole_adjoint(*args, 8, need, 0, None) == -2
 
