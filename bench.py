@@ -303,7 +303,7 @@ def bench_cylspline(dev, n: int = 1_000_000):
 
 
 def bench_high_order(dev, n: int = 10_000_000):
-    """Multipole lmax 32 (80 nodes, 1085 active harmonics), evalDeriv at
+    """Multipole lmax 32 (80 nodes, 1088 active harmonics), evalDeriv at
     N=10M points log-uniform in r over [0.05, 800] kpc."""
     import torch
     from agama_gpu_amd import MultipolePotentialGPU
