@@ -226,10 +226,46 @@ __device__ __forceinline__ void cyl_outer(const CSK &P, double x, double y, doub
     }
     o.phi = Sv;
     if (!G && !H) return;
+    if (H && pole) {
+        // The Hessian on the z axis (sin(theta) < 1e-10): the regular limit,
+        // as for the Multipole kernels (the chain rule's pole branch drops
+        // d2Y/dtheta2 and the |m| = 1, 2 limits).  A term is
+        // W (r0/r)^(l+1) mul Qh(z/r) h_m(x, y) / r^m with Qh = P~_lm / sin^m
+        // (the recurrence from the seed without its sin factors, at
+        // cos(theta) = s), V its value: m = 0 gives H_zz = (l+1)(l+2) V / r^2
+        // and H_xx = H_yy = -(l+1)(l+2) V / (2 r^2), m = 1 gives H_xz (cos),
+        // H_yz (sin) = -s (l+2) V / r^2, m = 2 gives H_xx = -H_yy = 2 V / r^2
+        // (cos) and H_xy = 2 V / r^2 (sin).
+        const double sg = ct < 0.0 ? -1.0 : 1.0;
+        double h0 = 0.0, hzz = 0.0, h2c = 0.0, h2s = 0.0, hxz = 0.0, hyz = 0.0;
+        double ph = 1.0 / sqrt(4.0 * M_PI);
+        for (int m = 0; m <= P.mmax_o && m <= 2; ++m) {
+            if (m > 0) ph *= kOuterRec.f[m];
+            const double mul = (m == 0) ? 2.0 * sqrt(M_PI) : 2.0 * sqrt(2.0 * M_PI);
+            double Q1 = ph, Q2 = 0.0, rl = q;
+            for (int l = 0; l < m; ++l) rl *= q;
+            for (int l = m; l <= P.lmax_o; ++l) {
+                if (l > m) {
+                    const double Qn = fma(kOuterRec.a[l][m] * sg, Q1, -kOuterRec.b[l][m] * Q2);
+                    Q2 = Q1; Q1 = Qn;
+                    rl *= q;
+                }
+                const double Y = mul * Q1 * rl, k = (double)(l + 2);
+                const double vc = cld(P.W, l * (l + 1) + m) * Y;
+                const double vs = (m > 0) ? cld(P.W, l * (l + 1) - m) * Y : 0.0;
+                if (m == 0) { hzz = fma((double)(l + 1) * k, vc, hzz); h0 = fma(-0.5 * (l + 1) * k, vc, h0); }
+                else if (m == 1) { hxz = fma(-sg * k, vc, hxz); hyz = fma(-sg * k, vs, hyz); }
+                else { h2c = fma(2.0, vc, h2c); h2s = fma(2.0, vs, h2s); }
+            }
+        }
+        const double ir2 = ir * ir;
+        o.h[0] = (h0 + h2c) * ir2; o.h[1] = (h0 - h2c) * ir2; o.h[2] = hzz * ir2;
+        o.h[3] = h2s * ir2; o.h[4] = hyz * ir2; o.h[5] = hxz * ir2;
+    }
     SphDerivs d;
     d.Pr = Sr; d.Pt = St; d.Pfos = pole ? (ct < 0.0 ? -Sfp : Sfp) : Sfn * ist;   // south pole: -
     d.Prr = Srr; d.Prt = Srt; d.Prf = Srf; d.Ptt = Stt; d.Ptf = Stf; d.Pf = Sf; d.Pff = Sff;
-    sph_to_cart<H>(x, y, z, ir, R, ct, st, cph, sph, d, o.g, o.h);
+    sph_to_cart<H>(x, y, z, ir, R, ct, st, cph, sph, d, o.g, o.h, H && pole);
 }
 
 // Outside the grid box in scaled coordinates (a NaN coordinate is inside:

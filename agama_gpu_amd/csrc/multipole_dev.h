@@ -414,7 +414,87 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
         d.Pf = lam * Sf; d.Pff = lam * Sff; d.Ptf = lam * Stf;
         d.Prf = fma(lam * idl, Srf, mu * Sf);
     }
-    sph_to_cart<ORD == ORD_HESS>(x, y, z, ir, R, ct, st, cph, sph, d, o.g, o.h);
+    if (ORD == ORD_HESS && pole) {
+        // ---- the Hessian on the z axis (sin(theta) < 1e-10) ------------------
+        // The spherical chain rule is singular there (and its pole branch
+        // drops d2Y/dtheta2 and the |m| = 1, 2 limits, as the reference's
+        // does).  On the axis only |m| <= 2 have second derivatives: with
+        // Qh = P~_lm / sin^m(theta), regular on the axis, a term is
+        // F(r) Qh(z/r) h_m(x, y) / r^m with h_0 = 1, h_1 = x or y, h_2 =
+        // x^2 - y^2 or 2xy, and in t = ln r, s = sign(cos theta), V its value:
+        //   m = 0: H_zz = (V_tt - V_t) / r^2, H_xx = H_yy = (V_t - l(l+1) V / 2) / r^2
+        //   m = 1: H_xz (cos), H_yz (sin) = s (V_t - V) / r^2
+        //   m = 2: H_xx = -H_yy = 2 V / r^2 (cos), H_xy = 2 V / r^2 (sin).
+        // Qh is the program's recurrence run from its seed without the
+        // sin^m factor, at cos(theta) = s.  The monopole and the log-scaling
+        // factors are formed again here (rarely taken; nothing is held in
+        // registers across the term loop for it); the chain rule below then
+        // forms only the gradient for these points.
+        double a0[6], c0, dc0, d2c0;
+        load6(0, a0);
+        horner(a0, c0, dc0, d2c0);
+        dc0 *= idl;
+        d2c0 *= idl * idl;
+        double lm = 1.0, mu = 0.0, nu = 0.0;
+        if (P.log_scaling) {
+            const double ex = agx_exp(c0), p0 = rcp_nr(P.invPhi0 - ex), dpdx = p0 * p0 * ex;
+            d2c0 = dpdx * fma(dc0 * dc0, p0 * (P.invPhi0 + ex), d2c0);
+            dc0 = dpdx * dc0;
+            c0 = p0;
+            lm = c0; mu = dc0; nu = d2c0;
+        }
+        const double sg = ct < 0.0 ? -1.0 : 1.0;
+        double v0 = 0.0, r0 = 0.0, rr0 = 0.0, l0 = 0.0;   // m = 0: sums of C, C', C'', l(l+1) C
+        double v1c = 0.0, r1c = 0.0, v1s = 0.0, r1s = 0.0, v2c = 0.0, v2s = 0.0;
+        const int S0p = P.n_groups + 1;
+        for (int g = 0; g < P.n_groups; ++g) {
+            const ivec8 gr = cld_rec(P.prog, g);
+            const int m = gr.s0;
+            if (m > 2) break;                       // groups come in increasing |m|
+            const double cc = (gr.s3 == 2) ? 1.0 : sg;   // cos^2 or cos theta
+            double Qm1 = 0.0, Qm2 = 0.0;
+            for (int j = gr.s1; j < gr.s2; ++j) {
+                const ivec8 rec = cld_rec(P.prog, S0p + j);
+                const double Q = (j == gr.s1)
+                    ? rec_lo(gr) : fma(rec_lo(rec) * (cc + rec_hi(rec)), Qm1, rec_mid(rec) * Qm2);
+                Qm2 = Qm1;
+                Qm1 = Q;
+                const int oc = rec_off_lo(rec), os = rec_off_hi(rec);
+                double a[6], c, dc, d2c;
+                if (oc >= 0) {
+                    load6(oc, a);
+                    horner(a, c, dc, d2c);
+                    if (m == 0) {
+                        v0 = fma(c, Q, v0); r0 = fma(dc, Q, r0); rr0 = fma(d2c, Q, rr0);
+                        l0 = fma((double)(rec.s0 * (rec.s0 + 1)) * c, Q, l0);
+                    } else if (m == 1) {
+                        v1c = fma(c, Q, v1c); r1c = fma(dc, Q, r1c);
+                    } else {
+                        v2c = fma(c, Q, v2c);
+                    }
+                }
+                if (SIN && os >= 0) {
+                    load6(os, a);
+                    horner(a, c, dc, d2c);
+                    if (m == 1) { v1s = fma(c, Q, v1s); r1s = fma(dc, Q, r1s); }
+                    else if (m == 2) { v2s = fma(c, Q, v2s); }
+                }
+            }
+        }
+        const double ir2 = ir * ir, li = lm * idl;
+        const double V0t = fma(li, r0, fma(mu, v0, dc0));
+        const double V0tt = fma(li * idl, rr0, fma(2.0 * mu * idl, r0, fma(nu, v0, d2c0)));
+        const double hxx0 = fma(-0.5 * lm, l0, V0t) * ir2;
+        const double q2c = 2.0 * lm * v2c * ir2;
+        o.h[0] = hxx0 + q2c;
+        o.h[1] = hxx0 - q2c;
+        o.h[2] = (V0tt - V0t) * ir2;
+        o.h[3] = 2.0 * lm * v2s * ir2;
+        o.h[4] = sg * (fma(li, r1s, mu * v1s) - lm * v1s) * ir2;
+        o.h[5] = sg * (fma(li, r1c, mu * v1c) - lm * v1c) * ir2;
+    }
+    sph_to_cart<ORD == ORD_HESS>(x, y, z, ir, R, ct, st, cph, sph, d, o.g, o.h,
+                                 ORD == ORD_HESS && pole);
 }
 
 // Classification of one point: 0 origin, 1 inner law, 2 interior (k), 3 outer law.

@@ -15,16 +15,18 @@ struct SphDerivs {
 // Chain rule of the reference Multipole kernel (_multipole_potential_kernel.cu
 // :390-409 gradient, :728-799 Hessian).  g: +grad Phi; h: [xx,yy,zz,xy,yz,xz].
 // On the z axis the phi terms are dropped (their coefficients vanish there).
+// keep_h: leave h as it is (the caller has formed it another way).
 template <bool HESS>
 __device__ __forceinline__ void sph_to_cart(double x, double y, double z, double ir, double R,
                                             double ct, double st, double cph, double sph,
-                                            const SphDerivs &d, double g[3], double h[6])
+                                            const SphDerivs &d, double g[3], double h[6],
+                                            bool keep_h = false)
 {
     const double ir2 = ir * ir;
     g[0] = fma(d.Pr * x, ir2, fma(d.Pt, ct * cph * ir, -d.Pfos * sph * ir));
     g[1] = fma(d.Pr * y, ir2, fma(d.Pt, ct * sph * ir, d.Pfos * cph * ir));
     g[2] = fma(d.Pr * z, ir2, -d.Pt * st * ir);
-    if (!HESS) return;
+    if (!HESS || keep_h) return;
     const double iR = (R > 1.0e-300) ? 1.0 / R : 0.0;
     const double c3[3] = {x, y, z};
     const double dl[3] = {x * ir2, y * ir2, z * ir2};

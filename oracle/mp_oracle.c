@@ -403,6 +403,62 @@ static void accum(const orc_mp *T, const term_ctx *X, acc_t *A, int ci, int l,
 
 static inline int iabs_(int v) { return v < 0 ? -v : v; }
 
+/* The Hessian on the z axis (sin(theta) < 1e-10), where the spherical chain
+ * rule below is singular and the reference's pole branch (d2Y = 0, kernel
+ * :610-620, and no |m| = 1, 2 limits) gives a wrong H_xx, H_yy, H_xy, H_xz and
+ * H_yz.  Only |m| <= 2 have second derivatives on the axis.  With
+ * Qh_lm = P~_lm / sin^m(theta) at u = cos(theta) = s = +-1 in closed form
+ * (Condon-Shortley phase, N_l = sqrt((2l+1)/4pi)):
+ *   m = 0: Qh = N_l s^l
+ *   m = 1: Qh = -N_l sqrt(l(l+1)) / 2 * s^(l+1)        (-N_l1 P_l'(s))
+ *   m = 2: Qh = N_l sqrt((l-1)l(l+1)(l+2)) / 8 * s^l    (N_l2 P_l''(s))
+ * a term is F(r) mul_m Qh(z/r) h_m(x, y) / r^m with h_0 = 1, h_1 = x (cos)
+ * or y (sin), h_2 = x^2 - y^2 (cos) or 2xy (sin); differentiating at x = y = 0
+ * with V = mul F Qh and t = ln r:
+ *   m = 0: H_zz = (V_tt - V_t)/r^2, H_xx = H_yy = (V_t - l(l+1) V / 2)/r^2
+ *   m = 1: H_xz (cos), H_yz (sin) = s (V_t - V)/r^2
+ *   m = 2: H_xx = -H_yy = 2 V / r^2 (cos), H_xy = 2 V / r^2 (sin). */
+static void pole_hessian(const orc_mp *T, const term_ctx *X, double r2, double *out_h)
+{
+    const double s = X->ct < 0.0 ? -1.0 : 1.0;
+    const double pf0 = MUL0 * g_prefact[0];
+    double v0t = pf0 * X->dC0, v0tt = pf0 * X->d2C0, l0 = 0.0;
+    double v1[2] = {0.0, 0.0}, r1[2] = {0.0, 0.0}, v2[2] = {0.0, 0.0};   /* [cos, sin] */
+    for (int ci = 1; ci < T->n_lm; ci++) {
+        const int l = T->lm_l[ci], m = T->lm_m[ci], am = iabs_(m);
+        if (am > 2) continue;
+        double cc, dcc, d2cc;
+        quintic(T->poly, ci, X->k, T->n_int, X->s, &cc, &dcc, &d2cc);
+        double dl_ = dcc * X->idl, d2l_ = d2cc * X->idl * X->idl;
+        double Cv = cc, dv = dl_, d2v = d2l_;
+        if (X->log_scaling) {
+            d2v = d2l_ * X->C0 + 2.0 * dl_ * X->dC0 + cc * X->d2C0;
+            dv = dl_ * X->C0 + cc * X->dC0;
+            Cv = cc * X->C0;
+        }
+        const double Nl = sqrt((2.0 * l + 1.0) / (4.0 * 3.14159265358979323846)), sl = (l % 2) ? s : 1.0;
+        double Qh;
+        if (am == 0) Qh = MUL0 * Nl * sl;
+        else if (am == 1) Qh = -MUL1 * Nl * sqrt((double)l * (l + 1)) * 0.5 * sl * s;
+        else Qh = MUL1 * Nl * sqrt((double)(l - 1) * l * (l + 1) * (l + 2)) * 0.125 * sl;
+        const int is_sin = m < 0;
+        if (am == 0) {
+            v0t += Qh * dv; v0tt += Qh * d2v; l0 += (double)(l * (l + 1)) * Qh * Cv;
+        } else if (am == 1) {
+            v1[is_sin] += Qh * Cv; r1[is_sin] += Qh * dv;
+        } else {
+            v2[is_sin] += Qh * Cv;
+        }
+    }
+    const double ir2 = 1.0 / r2, hxx0 = (v0t - 0.5 * l0) * ir2;
+    out_h[0] = hxx0 + 2.0 * v2[0] * ir2;
+    out_h[1] = hxx0 - 2.0 * v2[0] * ir2;
+    out_h[2] = (v0tt - v0t) * ir2;
+    out_h[3] = 2.0 * v2[1] * ir2;
+    out_h[4] = s * (r1[1] - v1[1]) * ir2;
+    out_h[5] = s * (r1[0] - v1[0]) * ir2;
+}
+
 /* One particle.  out_g: gradient (+grad Phi), out_h: [Hxx,Hyy,Hzz,Hxy,Hyz,Hxz] */
 static void eval_one(const orc_mp *T, double px, double py, double pz, int mode,
                      double *out_p, double *out_g, double *out_h, double inv4piG)
@@ -523,6 +579,7 @@ static void eval_one(const orc_mp *T, double px, double py, double pz, int mode,
     out_g[1] = A.Pr * (py * ir2) + A.Pt * (ct * sphi * ir) + A.Pfos * (cphi * ir);
     out_g[2] = A.Pr * (pz * ir2) + A.Pt * (-st * ir);
     if (mode != ORC_HESS) return;
+    if (X.near_pole) { pole_hessian(T, &X, r2, out_h); return; }
     /* full Cartesian chain rule -- reference :728-799 */
     double iR = R > 1e-300 ? 1.0 / R : 0.0;
     double dlr[3] = {px * ir2, py * ir2, pz * ir2};

@@ -139,3 +139,23 @@ def test_cylspline_grid_beyond_lds():
     Pp, gp, _, _ = A.plummer(x, 1.0 / A.G, 0.5)
     assert np.max(np.abs(pc - (P + Pp)) / np.abs(P + Pp)) < 1e-12
     assert np.max(np.abs(Fc + g + gp)) <= 1e-11 * np.abs(g + gp).max()
+
+
+def test_cylspline_outer_on_axis(setup):
+    """Points on the z axis (and within sin(theta) < 1e-10 of it) beyond the
+    grid box, where the outer multipole's Hessian is its regular limit: equal
+    to the oracle's (a central difference of its complex-step gradient), at
+    both poles; the chain rule's pole branch (d2Y/dtheta2 = 0, no |m| = 1, 2
+    limits) was off there."""
+    gpu, orc = setup
+    z = np.array([600.0, -600.0, 2000.0, -2500.0])
+    x = np.concatenate([np.stack([0 * z, 0 * z, z], axis=1),
+                        np.stack([3e-12 * np.abs(z), 0 * z, z], axis=1)])
+    print("cylspline outer on axis", flush=True)
+    p, F, D = [t.cpu().numpy() for t in gpu.evalDeriv(torch.as_tensor(x, device="cuda"))]
+    P, g, H, outer = orc.eval(x)
+    assert outer.all()
+    assert np.max(np.abs(p - P) / np.abs(P)) < 1e-12
+    assert np.max(np.abs(F + g)) <= 1e-11 * np.abs(g).max()
+    for i in range(len(x)):
+        assert np.max(np.abs(D[i] + H[i])) <= 1e-6 * np.abs(H[i]).max(), (x[i], D[i], -H[i])

@@ -194,9 +194,27 @@ def test_near_axis_force(fname, mp_path):
     # phi varies by O(eps) off the axis (the m = 1 terms)
     assert np.max(np.abs(p - po) / np.abs(po)) < 1e-6
     assert np.max(np.abs(F - Fo).max(1) / np.abs(Fo).max(1)) < 1e-5
-    pr, Fr = OracleMultipole.from_coefs(c).pot_force(on)
+    orc = OracleMultipole.from_coefs(c)
+    pr, Fr = orc.pot_force(on)
     assert np.max(np.abs(po - pr) / np.abs(pr)) < 1e-12
     assert np.max(np.abs(Fo - Fr)) < 1e-11 * np.abs(Fr).max()
+    # Hessian: on the axis, and within sin(theta) < 1e-10 of it, the regular
+    # limit (equal to the oracle's closed form, itself pinned by differences of
+    # the force); between 1e-10 and 1e-6 the spherical chain rule, whose
+    # cancelling 1/sin(theta) terms leave an error of order 1e-16 / sin^2
+    # (theta) of max|H|: finite, and that close to the axis value
+    D = gpu.evalDeriv(torch.as_tensor(x, device=DEV))[2].cpu().numpy()
+    Do = gpu.evalDeriv(torch.as_tensor(on, device=DEV))[2].cpu().numpy()
+    Dr = orc.eval_deriv(on)[2]
+    assert np.isfinite(D).all()
+    assert np.max(np.abs(Do - Dr)) < 1e-10 * np.abs(Dr).max()
+    rel = np.abs(D - Do).max(1) / np.abs(Do).max(1)
+    pole = eps < 1e-10
+    print(f"  pole rows max rel {rel[pole].max():.2e}, band max rel*eps^2 "
+          f"{(rel * eps ** 2)[~pole].max():.2e}, band max rel/eps {(rel / eps)[~pole].max():.2e}",
+          flush=True)
+    assert rel[pole].max() < 1e-8
+    assert np.all(rel[~pole] < 1e-4 + 1e-15 / eps[~pole] ** 2)      # measured: 7e-17 / eps^2
 
 
 def _bad_points():

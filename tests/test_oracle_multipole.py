@@ -226,6 +226,45 @@ def test_origin_and_pole_branches():
         assert np.abs(Fo[0, :2]).max() > 1e-3 * abs(Fo[0, 2])     # m = 1 terms present
 
 
+def _fd_hessian(o, x0, h):
+    """+Hessian at x0 (1, 3) by a 5-point central difference of the force."""
+    H = np.zeros((3, 3))
+    for j in range(3):
+        e = np.zeros((1, 3))
+        e[0, j] = h
+        F = [o.pot_force(x0 + k * e)[1][0] for k in (2, 1, -1, -2)]
+        H[:, j] = -(-F[0] + 8 * F[1] - 8 * F[2] + F[3]) / (12 * h)
+    return H
+
+
+def _as_matrix(D):
+    """deriv [xx, yy, zz, xy, yz, xz] (= -Hessian) -> +Hessian 3 x 3."""
+    return -np.array([[D[0], D[3], D[5]], [D[3], D[1], D[4]], [D[5], D[4], D[2]]])
+
+
+@pytest.mark.parametrize("fname", FILES + ["halo"])
+def test_hessian_on_the_axis(fname):
+    """On the z axis (and within sin(theta) < 1e-10 of it) the Hessian is the
+    regular limit: it equals a 5-point difference of the force, which is
+    evaluated off the axis by the ordinary spherical chain rule, at both
+    poles; the reference's pole branch (d2Y/dtheta2 = 0, no |m| = 1, 2 limits)
+    was 10-40 % of max|H| off here.  H_xx != H_yy and H_xy, H_xz, H_yz != 0
+    where m = 1, 2 harmonics are present."""
+    if fname == "halo":
+        from agama_gpu_amd.workloads import halo_lmax8_coefs
+        o = OracleMultipole.from_coefs(halo_lmax8_coefs())
+    else:
+        o = OracleMultipole.from_coefs(read_coefs(FIXTURES / fname))
+    for z in (0.7, 5.0, -5.0, 30.0, -120.0):
+        for eps in (0.0, 3e-12, 8e-11):
+            x0 = np.array([[eps * abs(z), 0.0, z]])
+            Hk = _as_matrix(o.eval_deriv(x0)[2][0])
+            Hf = _fd_hessian(o, np.array([[0.0, 0.0, z]]), 1e-3 * abs(z))
+            assert np.max(np.abs(Hk - Hf)) < 1e-7 * np.abs(Hf).max(), (fname, z, eps)
+            if fname != "halo":
+                assert np.abs(Hf[0, 0] - Hf[1, 1]) > 1e-4 * np.abs(Hf).max()
+
+
 # ---- extrapolation laws: tables whose continuation beyond the grid is fixed ----
 LAW_CASES = ["inner_log", "inner_core", "outer_log", "kepler", "outer_power"]
 
