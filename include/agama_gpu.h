@@ -247,7 +247,7 @@ int agx_multipole_pack(const agx_multipole *mp, void *dst, int64_t size);
  * product) and ADDS dL/d(poly_km) into d_gpoly [n_int][n_lm][6] and
  * dL/d(invPhi0, inner_U, inner_W, outer_U, outer_W) into d_gscal [5]; the
  * caller zeroes those first.  d_work is device scratch of at least
- * agx_multipole_work_size(N) bytes (about 10 N; the particles are
+ * agx_multipole_work_size(N) bytes (about 6 N; the particles are
  * sorted by radial interval there); it is overwritten and needs no clearing.
  * flags: AGX_WORK_SORTED if d_work already holds the sort of these d_xyz
  * for this radial grid (left by agx_multipole_eval_sorted or an earlier
