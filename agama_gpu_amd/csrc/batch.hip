@@ -97,49 +97,38 @@ __global__ __launch_bounds__(kBatchBlock) void mp_batch_kernel(
     }
 }
 
+// Grid of the persistent kernel: the workgroups resident at once on the current
+// device (cached per device by resident_workgroups), at most one per work item.
+// The table's dynamic LDS exceeds 64 KB, which has to be allowed per kernel
+// first; 0 if that fails, so the caller reports a launch error.
 template <int ORD, int STRIDE, bool SIN>
 static unsigned batch_grid(int64_t n_items)
 {
-    static int per_cu[4] = {0, 0, 0, 0};
-    static int n_cu = 0;
     const size_t lds = (size_t)kBatchTableDoubles * sizeof(double);
-    if (!n_cu) {
-        int dev = 0, v = 0;
-        if (hipGetDevice(&dev) == hipSuccess &&
-            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
-            n_cu = v;
-        else
-            n_cu = 256;
-    }
-    if (!per_cu[ORD]) {
-        hipFuncSetAttribute((const void *)mp_batch_kernel<ORD, STRIDE, SIN>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        int b = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, mp_batch_kernel<ORD, STRIDE, SIN>,
-                                                         kBatchBlock, lds) != hipSuccess ||
-            b < 1)
-            b = 1;
-        per_cu[ORD] = b;
-    }
-    int64_t g = (int64_t)n_cu * per_cu[ORD];
+    const void *k = (const void *)mp_batch_kernel<ORD, STRIDE, SIN>;
+    if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return 0;
+    int64_t g = resident_workgroups(k, kBatchBlock, lds, 256);
     if (g > n_items) g = n_items;
     if (g < 1) g = 1;
     return (unsigned)g;
 }
 
 template <int ORD, int STRIDE, bool SIN>
-static void launch_batch(const char *descs, int64_t stride, int n_pot, const agx_batch_item *items,
+static bool launch_batch(const char *descs, int64_t stride, int n_pot, const agx_batch_item *items,
                          int64_t n_items, const double *xyz, int64_t n_rows, double *phi,
                          double *force, double *deriv, int *queue, hipStream_t s)
 {
-    hipLaunchKernelGGL((mp_batch_kernel<ORD, STRIDE, SIN>),
-                       dim3(batch_grid<ORD, STRIDE, SIN>(n_items)), dim3(kBatchBlock),
+    const unsigned grid = batch_grid<ORD, STRIDE, SIN>(n_items);
+    if (!grid) return false;
+    hipLaunchKernelGGL((mp_batch_kernel<ORD, STRIDE, SIN>), dim3(grid), dim3(kBatchBlock),
                        kBatchTableDoubles * sizeof(double), s, descs, stride, n_pot, items, n_items,
                        xyz, n_rows, phi, force, deriv, queue);
+    return true;
 }
 
 template <int ORD>
-static void launch_batch(uint32_t flags, const char *descs, int64_t stride, int n_pot,
+static bool launch_batch(uint32_t flags, const char *descs, int64_t stride, int n_pot,
                          const agx_batch_item *items, int64_t n_items, const double *xyz,
                          int64_t n_rows, double *phi, double *force, double *deriv, int *queue,
                          hipStream_t s)
@@ -147,14 +136,14 @@ static void launch_batch(uint32_t flags, const char *descs, int64_t stride, int 
     const int st = prog_stride(flags);
     const bool co = flags & AGX_PROG_COS_ONLY;
 #define AGX_LAUNCH_BATCH(S, C)                                                                  \
-    launch_batch<ORD, S, C>(descs, stride, n_pot, items, n_items, xyz, n_rows, phi, force, deriv, \
-                            queue, s)
+    return launch_batch<ORD, S, C>(descs, stride, n_pot, items, n_items, xyz, n_rows, phi, force, \
+                                   deriv, queue, s)
     if (st == 1 && co) AGX_LAUNCH_BATCH(1, false);
-    else if (st == 1) AGX_LAUNCH_BATCH(1, true);
-    else if (st == 2 && co) AGX_LAUNCH_BATCH(2, false);
-    else if (st == 2) AGX_LAUNCH_BATCH(2, true);
-    else if (co) AGX_LAUNCH_BATCH(0, false);
-    else AGX_LAUNCH_BATCH(0, true);
+    if (st == 1) AGX_LAUNCH_BATCH(1, true);
+    if (st == 2 && co) AGX_LAUNCH_BATCH(2, false);
+    if (st == 2) AGX_LAUNCH_BATCH(2, true);
+    if (co) AGX_LAUNCH_BATCH(0, false);
+    AGX_LAUNCH_BATCH(0, true);
 #undef AGX_LAUNCH_BATCH
 }
 
@@ -229,14 +218,15 @@ extern "C" int agx_multipole_batch_eval(const void *d_desc, int64_t desc_stride,
     const char *descs = (const char *)d_desc;
     hipLaunchKernelGGL(batch_queue_reset, dim3(1), dim3(64), 0, s, (int *)d_queue);
     int *q = (int *)d_queue;
+    bool ok;
     if (deriv)
-        launch_batch<ORD_HESS>(prog_flags, descs, desc_stride, n_pot, d_items, n_items, d_xyz,
-                               n_rows, phi, force, deriv, q, s);
+        ok = launch_batch<ORD_HESS>(prog_flags, descs, desc_stride, n_pot, d_items, n_items, d_xyz,
+                                    n_rows, phi, force, deriv, q, s);
     else if (force)
-        launch_batch<ORD_GRAD>(prog_flags, descs, desc_stride, n_pot, d_items, n_items, d_xyz,
-                               n_rows, phi, force, nullptr, q, s);
+        ok = launch_batch<ORD_GRAD>(prog_flags, descs, desc_stride, n_pot, d_items, n_items, d_xyz,
+                                    n_rows, phi, force, nullptr, q, s);
     else
-        launch_batch<ORD_POT>(prog_flags, descs, desc_stride, n_pot, d_items, n_items, d_xyz,
-                              n_rows, phi, nullptr, nullptr, q, s);
-    return hipGetLastError() == hipSuccess ? AGX_OK : AGX_E_LAUNCH;
+        ok = launch_batch<ORD_POT>(prog_flags, descs, desc_stride, n_pot, d_items, n_items, d_xyz,
+                                   n_rows, phi, nullptr, nullptr, q, s);
+    return (hipGetLastError() == hipSuccess && ok) ? AGX_OK : AGX_E_LAUNCH;
 }
