@@ -304,6 +304,40 @@ __device__ __forceinline__ void cylspline_point(const CSK &P, double x, double y
         cylspline_inside<ORD>(P, x, y, z, cc, o);
 }
 
+// One harmonic's bicubic spline in cell (iR, iz) at (tR, tz): its value v,
+// dv/dlR (G), d2v/dlR2 (H) and, with Z, dv/dlz (G), d2v/dlRdlz, d2v/dlz2 (H).
+// topR / topZ: the point is on the cell's upper lR / lz node.
+template <bool G, bool H, bool Z>
+__device__ __forceinline__ void harm_cell(const CSK &P, int mm, int iR, int iz, double tR,
+                                          double tz, double hR, double hz, double ihR, double ihz,
+                                          bool topR, bool topZ, double &v, double &vR, double &vRR,
+                                          double &vZ, double &vRZ, double &vZZ)
+{
+    const int64_t plane = (int64_t)P.nR * P.nz;
+    // corners (iR,iz), (iR,iz+1) and (iR+1,iz), (iR+1,iz+1): two 64-byte runs
+    const dvec2 *lo = (const dvec2 *)(P.nodes + ((int64_t)mm * plane + (int64_t)iR * P.nz + iz) * 4);
+    const dvec2 *hi = lo + 2 * P.nz;
+    const dvec2 a0 = lo[0], a1 = lo[1], b0 = lo[2], b1 = lo[3];   // (iR,iz): f,fx | fy,fxy ; (iR,iz+1)
+    const dvec2 c0 = hi[0], c1 = hi[1], e0 = hi[2], e1 = hi[3];   // (iR+1,iz) ; (iR+1,iz+1)
+    // offset by one corner value for accuracy (reference :551-554)
+    const double off = topR ? (topZ ? e0.x : c0.x) : (topZ ? b0.x : a0.x);
+    // step 1: splines in lz of (f, f at iR+1, fx, fx at iR+1)
+    double Fz[4], dFz[4], d2Fz[4];
+    const double fl[4] = {a0.x - off, c0.x - off, a0.y, c0.y};
+    const double fu[4] = {b0.x - off, e0.x - off, b0.y, e0.y};
+    const double gl[4] = {a1.x, c1.x, a1.y, c1.y};
+    const double gu[4] = {b1.x, e1.x, b1.y, e1.y};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        herm1<Z && G, Z && H>(tz, hz, ihz, fl[k], fu[k], gl[k], gu[k], Fz[k], dFz[k], d2Fz[k]);
+    // step 2: splines in lR
+    double dummy;
+    herm1<G, H>(tR, hR, ihR, Fz[0], Fz[1], Fz[2], Fz[3], v, vR, vRR);
+    v += off;
+    if (Z && G) herm1<H, false>(tR, hR, ihR, dFz[0], dFz[1], dFz[2], dFz[3], vZ, vRZ, dummy);
+    if (Z && H) herm1<false, false>(tR, hR, ihR, d2Fz[0], d2Fz[1], d2Fz[2], d2Fz[3], vZZ, dummy, dummy);
+}
+
 // The bicubic spline of a point inside the box.  LDS: the grids are read from
 // the LDS copies gR (nR nodes) and gz (nz nodes) that stage_grids made;
 // otherwise from the table.
@@ -331,31 +365,12 @@ __device__ __forceinline__ void cylspline_inside(const CSK &P, double x, double 
     double F0 = 0, F0R = 0, F0Z = 0, F0RR = 0, F0RZ = 0, F0ZZ = 0;
     int cur_m = 0;
     double cm = 1.0, sm = 0.0;
-    const int64_t plane = (int64_t)P.nR * P.nz;
+    const bool topR = lR == lR1, topZ = lz == lz1;
     for (int mm = 0; mm < P.n_harm; ++mm) {
         const int mv = cldi(P.m_arr, mm), am = mv < 0 ? -mv : mv;
-        // corners (iR,iz), (iR,iz+1) and (iR+1,iz), (iR+1,iz+1): two 64-byte runs
-        const dvec2 *lo = (const dvec2 *)(P.nodes + ((int64_t)mm * plane + (int64_t)iR * P.nz + iz) * 4);
-        const dvec2 *hi = lo + 2 * P.nz;
-        const dvec2 a0 = lo[0], a1 = lo[1], b0 = lo[2], b1 = lo[3];   // (iR,iz): f,fx | fy,fxy ; (iR,iz+1)
-        const dvec2 c0 = hi[0], c1 = hi[1], e0 = hi[2], e1 = hi[3];   // (iR+1,iz) ; (iR+1,iz+1)
-        // offset by one corner value for accuracy (reference :551-554)
-        const double off = (lR == lR1) ? ((lz == lz1) ? e0.x : c0.x) : ((lz == lz1) ? b0.x : a0.x);
-        // step 1: splines in lz of (f, f at iR+1, fx, fx at iR+1)
-        double Fz[4], dFz[4], d2Fz[4];
-        const double fl[4] = {a0.x - off, c0.x - off, a0.y, c0.y};
-        const double fu[4] = {b0.x - off, e0.x - off, b0.y, e0.y};
-        const double gl[4] = {a1.x, c1.x, a1.y, c1.y};
-        const double gu[4] = {b1.x, e1.x, b1.y, e1.y};
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            herm1<G || H, H>(tz, hz, ihz, fl[k], fu[k], gl[k], gu[k], Fz[k], dFz[k], d2Fz[k]);
-        // step 2: splines in lR
-        double v, vR = 0, vRR = 0, vZ = 0, vRZ = 0, vZZ = 0, dummy;
-        herm1<G || H, H>(tR, hR, ihR, Fz[0], Fz[1], Fz[2], Fz[3], v, vR, vRR);
-        v += off;
-        if (G || H) herm1<H, false>(tR, hR, ihR, dFz[0], dFz[1], dFz[2], dFz[3], vZ, vRZ, dummy);
-        if (H) herm1<false, false>(tR, hR, ihR, d2Fz[0], d2Fz[1], d2Fz[2], d2Fz[3], vZZ, dummy, dummy);
+        double v, vR = 0, vRR = 0, vZ = 0, vRZ = 0, vZZ = 0;
+        harm_cell<G || H, H, true>(P, mm, iR, iz, tR, tz, hR, hz, ihR, ihz, topR, topZ, v, vR, vRR,
+                                   vZ, vRZ, vZZ);
         if (mv == 0) {
             F0 = v; F0R = vR; F0Z = vZ; F0RR = vRR; F0RZ = vRZ; F0ZZ = vZZ;
             if (!P.log_scaling) {
@@ -382,6 +397,33 @@ __device__ __forceinline__ void cylspline_inside(const CSK &P, double x, double 
         if (H) {
             SRR = fma(vRR, T, SRR); SRZ = fma(vRZ, T, SRZ); SZZ = fma(vZZ, T, SZZ);
             SRp = fma(vR, dT, SRp); SZp = fma(vZ, dT, SZp); Spp = fma(v, d2T, Spp);
+        }
+    }
+    // ---- on the z axis: the regular part of d2Phi/dlR2 -----------------------
+    // Every harmonic's spline has zero slope in lR at the axis, so near it
+    // Phi = Phi(axis) + lR^2 / 2 sum_m Q_m T_m(phi) + O(lR^3), lR ~ R / Rscale.
+    // Of the lR^2 terms only m = 0 (x^2 + y^2), m = +2 (x^2 - y^2) and m = -2
+    // (2 x y) are smooth in x, y; the others' second derivatives depend on the
+    // direction of approach and average to zero over it, which is the limit
+    // taken here.  Q_m is the phi-Fourier coefficient of d2Phi/dlR2 at the
+    // point (with log-scaling the product rule of Phi = -exp(F0) S).
+    double axQ0 = 0.0, axQc = 0.0, axQs = 0.0;
+    if (H && !(R > 1.0e-30)) {
+        double Pa = 1.0, PaR = 0.0, PaRR = 0.0;
+        if (P.log_scaling) {
+            Pa = -agx_exp(F0); PaR = Pa * F0R; PaRR = Pa * (F0RR + F0R * F0R);
+            axQ0 = PaRR;
+        } else {
+            axQ0 = F0RR;
+        }
+        for (int mm = 0; mm < P.n_harm; ++mm) {
+            const int mv = cldi(P.m_arr, mm);
+            if (mv != 2 && mv != -2) continue;
+            double v, vR, vRR, d0, d1, d2;
+            harm_cell<true, true, false>(P, mm, iR, iz, tR, tz, hR, hz, ihR, ihz, topR, topZ, v, vR,
+                                         vRR, d0, d1, d2);
+            const double q = fma(Pa, vRR, fma(2.0 * PaR, vR, PaRR * v));
+            if (mv > 0) axQc = q; else axQs = q;
         }
     }
     // ---- physical derivatives in (lR, lz, phi) ------------------------------
@@ -427,7 +469,8 @@ __device__ __forceinline__ void cylspline_inside(const CSK &P, double x, double 
     }
     if (ORD == ORD_DENS) {
         // Laplacian in cylindrical coordinates: Phi_RR + Phi_R/R + Phi_pp/R^2 + Phi_zz
-        const double lap = (R > 1.0e-30) ? HRR + (PhR + Ppp * invR) * invR + HZZ : 2.0 * HRR + HZZ;
+        const double lap = (R > 1.0e-30) ? HRR + (PhR + Ppp * invR) * invR + HZZ
+                                         : 2.0 * axQ0 * jR * jR + HZZ;
         o.phi = lap * (1.0 / (4.0 * M_PI * G_AGAMA_D));
         return;
     }
@@ -451,7 +494,11 @@ __device__ __forceinline__ void cylspline_inside(const CSK &P, double x, double 
         o.h[4] = HRZ * sph + HZp * cph * invR;
         o.h[5] = HRZ * cph - HZp * sph * invR;
     } else {
-        o.h[0] = HRR; o.h[1] = HRR; o.h[3] = 0.0; o.h[4] = 0.0; o.h[5] = 0.0;
+        // the slopes in lR vanish on the axis, and with them the m = +-1 part of
+        // d2Phi/dRdz: H_xz = H_yz = 0
+        const double j2 = jR * jR;
+        o.h[0] = j2 * (axQ0 + axQc); o.h[1] = j2 * (axQ0 - axQc); o.h[3] = j2 * axQs;
+        o.h[4] = 0.0; o.h[5] = 0.0;
     }
     o.h[2] = HZZ;
 }

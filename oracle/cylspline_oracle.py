@@ -221,6 +221,7 @@ class OracleCylSpline:
         S = np.ones(len(x)) if self.log_scaling else np.zeros(len(x))
         acc = {k: np.zeros(len(x)) for k in ("R", "Z", "RR", "RZ", "ZZ", "p", "pp", "Rp", "Zp")}
         F0 = None
+        q2 = {}
         for m, (f, fx, fy, fxy) in self.harm.items():
             fll, flu, ful, fuu = f[iR, iz], f[iR, iz + 1], f[iR + 1, iz], f[iR + 1, iz + 1]
             off = np.where(lR == xh, np.where(lz == yh, fuu, ful), np.where(lz == yh, flu, fll))
@@ -233,6 +234,8 @@ class OracleCylSpline:
             v = v + off
             vZ, vRZ, _ = c1(tR, hR, Fz[0][1], Fz[1][1], Fz[2][1], Fz[3][1])
             vZZ, _, _ = c1(tR, hR, Fz[0][2], Fz[1][2], Fz[2][2], Fz[3][2])
+            if m in (2, -2):
+                q2[m] = (v, vR, vRR)
             if m == 0:
                 F0 = (v, vR, vZ, vRR, vRZ, vZZ)
                 if not self.log_scaling:
@@ -289,10 +292,32 @@ class OracleCylSpline:
         hxy = HRR * c * s + HRp * (c * c - s * s) * iR_ - dR * iR_ * c * s - Pp * (c * c - s * s) * iR2 - Ppp * c * s * iR2
         hyz = HRZ * s + HZp * c * iR_
         hxz = HRZ * c - HZp * s * iR_
+        # On the z axis every harmonic's spline has zero slope in lR, so
+        # Phi = Phi(axis) + lR^2/2 sum_m Q_m T_m(phi) + O(lR^3) with Q_m the
+        # Fourier coefficients of d2Phi/dlR2; the m = 0, +2, -2 terms are
+        # (x^2 + y^2, x^2 - y^2, 2xy) Q_m / (2 Rscale^2) to second order, the
+        # others depend on the direction of approach and average out over it.
+        # (The reference's R = 0 branch, _cylspl_potential_kernel.cu:855-863,
+        # puts the whole d2Phi/dR2 at phi = 0 on both diagonal entries.)
         on = R <= 1e-30
-        hxx = np.where(on, HRR, hxx)
-        hyy = np.where(on, HRR, hyy)
-        hxy = np.where(on, 0.0, hxy)
+        if self.log_scaling:
+            Pa, PaR = P0, P0R
+            PaRR = P0 * (v0RR + v0R ** 2)
+            Q0 = PaRR
+        else:
+            Pa, PaR, PaRR = 1.0, 0.0, 0.0
+            Q0 = v0RR
+        Qm = {}
+        for m in (2, -2):
+            if m in q2:
+                v2, v2R, v2RR = q2[m]
+                Qm[m] = Pa * v2RR + 2 * PaR * v2R + PaRR * v2
+            else:
+                Qm[m] = 0.0
+        j2 = jR ** 2
+        hxx = np.where(on, j2 * (Q0 + Qm[2]), hxx)
+        hyy = np.where(on, j2 * (Q0 - Qm[2]), hyy)
+        hxy = np.where(on, j2 * Qm[-2], hxy)
         hyz = np.where(on, 0.0, hyz)
         hxz = np.where(on, 0.0, hxz)
         return Pv, g, np.stack([hxx, hyy, HZZ, hxy, hyz, hxz], axis=1)

@@ -175,3 +175,15 @@ def exterior_coefs(p=EXT):
     s = np.maximum(1.0, 2.0 / np.hypot(RR, ZZ))
     tabs = exterior_harmonics(RR * s, ZZ * s, p)
     return CylSplineCoefs(R, z, {m: np.asarray(t, float) for m, t in tabs.items()}, 2)
+
+
+def axis_tables():
+    """Exterior fields with m = 0 and m = +-2 harmonics only (no dipole), so
+    the spline's potential is twice differentiable across the z axis:
+    log-scaled (every m = 0 node negative) and, shifted by a constant,
+    linear."""
+    from agama_gpu_amd.coefs import CylSplineCoefs
+    c = exterior_coefs(dict(EXT, D=0.0))
+    lin = CylSplineCoefs(c.R_grid, c.z_grid,
+                         {m: t + (0.3 if m == 0 else 0.0) for m, t in c.phi.items()}, c.mmax)
+    return {"log": c, "linear": lin}

@@ -159,3 +159,49 @@ def test_cylspline_outer_on_axis(setup):
     assert np.max(np.abs(F + g)) <= 1e-11 * np.abs(g).max()
     for i in range(len(x)):
         assert np.max(np.abs(D[i] + H[i])) <= 1e-6 * np.abs(H[i]).max(), (x[i], D[i], -H[i])
+
+
+@pytest.mark.parametrize("table", ["bar", "m02_log", "m02_linear"])
+def test_cylspline_inner_on_axis(setup, cyl_path, table):
+    """Points exactly on the z axis inside the grid (R = 0, and R = 1e-31
+    where 1/R is still finite) mixed into random points: the Hessian is the
+    regular limit (the m = 0 and m = +-2 terms' lR^2 parts; the other
+    harmonics' average over directions), equal to the oracle's, which the
+    oracle tests pin by differences across the axis; the density equals the
+    Hessian's trace / 4 pi G; the same as a composite leaf."""
+    from agama_gpu_amd import CompositePotentialGPU, CylSplinePotentialGPU, PotentialGPU
+    from oracle import analytic_oracle as A
+    from oracle.cylspline_oracle import OracleCylSpline
+    if table == "bar":
+        gpu, orc = setup
+    else:
+        from exact_fields import axis_tables
+        c = axis_tables()[table[4:]]
+        gpu, orc = CylSplinePotentialGPU(c), OracleCylSpline(c)
+    rng = np.random.default_rng(11)
+    n = 4096
+    zmax = orc.Rscale * np.sinh(orc.lz[-1])
+    z = rng.uniform(-0.9, 0.9, n) * zmax
+    ph = rng.uniform(0, 2 * np.pi, n)
+    Rr = orc.Rscale * np.sinh(rng.uniform(orc.lR[0], orc.lR[-1], n))
+    R = np.where(np.arange(n) % 3 == 0, 0.0, np.where(np.arange(n) % 3 == 1, 1e-31, Rr))
+    x = np.stack([R * np.cos(ph), R * np.sin(ph), z], axis=1)
+    print(f"cylspline inner on axis table={table} path={cyl_path}", flush=True)
+    P, g, H, outer = orc.eval(x)
+    assert not outer.any()
+    xt = torch.as_tensor(x, device="cuda")
+    p, F, D = [t.cpu().numpy() for t in gpu.evalDeriv(xt)]
+    assert np.max(np.abs(p - P) / np.abs(P)) < 1e-12
+    assert np.max(np.abs(F + g)) <= 1e-11 * np.abs(g).max()
+    axis = R < 1e-30
+    for sel in (axis, ~axis):
+        assert np.max(np.abs(D[sel] + H[sel])) <= 1e-10 * np.abs(H[sel]).max()
+    rho = gpu.density(xt).cpu().numpy()
+    tr = (H[:, 0] + H[:, 1] + H[:, 2]) / (4 * np.pi * A.G)
+    assert np.max(np.abs(rho - tr)) <= 1e-9 * np.abs(tr).max()
+    comp = CompositePotentialGPU([gpu, PotentialGPU(type="Plummer", mass=1.0 / A.G,
+                                                    scaleRadius=0.5)])
+    pc, Fc, Dc = [t.cpu().numpy() for t in comp.evalDeriv(xt)]
+    Pp, gp, Hp, _ = A.plummer(x, 1.0 / A.G, 0.5)
+    assert np.max(np.abs(Dc[axis] + H[axis] + Hp[axis])) <= 1e-10 * np.abs(H[axis] + Hp[axis]).max()
+    assert np.max(np.abs(Fc + g + gp)) <= 1e-11 * np.abs(g + gp).max()

@@ -165,3 +165,64 @@ def test_rscale_and_outer_values_follow():
         pout, _, _, oout = o.eval(base * (1 + eps))
         assert not oin.any() and oout.all()
         assert np.max(np.abs(pin - pout) / np.abs(pout)) < 1e-8
+
+
+# ---- exactly on the z axis ---------------------------------------------------
+@pytest.mark.parametrize("kind", ["log", "linear"])
+def test_axis_hessian_is_the_limit(kind):
+    """On the axis the Hessian equals central differences of the off-axis
+    gradient across it, Richardson-extrapolated from steps of 1e-5 and
+    5e-6 Rscale along x, y, z (the spline's |R|^3 terms make one difference
+    first-order accurate): H_xx and H_yy differ by the m = 2 terms and H_xy
+    is the m = -2 term, which the reference's R = 0 branch (H_xx = H_yy =
+    d2Phi/dR2 at phi = 0, H_xy = 0; _cylspl_potential_kernel.cu:855-863)
+    misses."""
+    from exact_fields import axis_tables
+    o = OracleCylSpline(axis_tables()[kind])
+    assert o.log_scaling == (kind == "log")
+    z = np.array([-60.0, -7.5, -0.3, 0.3, 1.0, 4.0, 25.0, 70.0])
+    on = np.stack([0 * z, 0 * z, z], axis=1)
+    P, g, H, outer = o.eval(on)
+    assert not outer.any()
+
+    def diffs(d):
+        gp = [o.eval(on + d * e)[1] for e in np.eye(3)]
+        gm = [o.eval(on - d * e)[1] for e in np.eye(3)]
+        # slots xx, yy, zz, xy, yz, xz as (component, direction)
+        return np.stack([(gp[j][:, i] - gm[j][:, i]) / (2 * d)
+                         for i, j in [(0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (0, 2)]], axis=1)
+
+    d = 1e-5 * o.Rscale
+    fd = 2 * diffs(d / 2) - diffs(d)
+    scale = np.abs(H).max()
+    assert np.max(np.abs(fd - H)) <= 1e-8 * scale
+    # the m = 2 terms are not negligible here: H_xx != H_yy, H_xy != 0
+    assert np.abs(H[:, 0] - H[:, 1]).max() > 1e-3 * scale
+    assert np.abs(H[:, 3]).max() > 1e-3 * scale
+
+
+def test_axis_hessian_is_the_direction_average(orc):
+    """For a table with every harmonic (the bar fixture: m = -8..8, zero on the
+    axis for m != 0), the Hessian of the m = +-1 and |m| >= 3 terms has no
+    limit at the axis -- it depends on the direction of approach -- and the
+    axis value is its average over directions: the mean of the off-axis
+    Hessian over 64 equally spaced azimuths, Richardson-extrapolated to R = 0
+    from R = 1e-6 and 5e-7 Rscale (the mean is R-linear at first order)."""
+    z = np.array([-40.0, -3.0, 0.2, 2.0, 9.0, 150.0])
+    on = np.stack([0 * z, 0 * z, z], axis=1)
+    P, g, H, outer = orc.eval(on)
+    assert not outer.any()
+    ph = 2 * np.pi * np.arange(64) / 64
+
+    def ring(R):
+        x = np.stack([R * np.tile(np.cos(ph), len(z)), R * np.tile(np.sin(ph), len(z)),
+                      np.repeat(z, 64)], axis=1)
+        return orc.eval(x)[2].reshape(len(z), 64, 6)
+
+    R = 1e-6 * orc.Rscale
+    Hr = ring(R)
+    scale = np.abs(H).max()
+    assert np.max(np.abs(2 * ring(R / 2).mean(1) - Hr.mean(1) - H)) <= 1e-9 * scale
+    # the directions do differ (the average is not a single direction's value)
+    assert np.abs(Hr[:, :, 0] - Hr[:, :1, 0]).max() > 1e-4 * scale
+    assert np.abs(H[:, 0] - H[:, 1]).max() > 1e-4 * scale
