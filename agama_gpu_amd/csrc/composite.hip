@@ -43,7 +43,7 @@ struct CompositeArgs {
 // kGridLDS) is evaluated point by point instead.  Every output is written
 // once.
 template <int ORD, int STRIDE, bool SIN>
-__global__ __launch_bounds__(kBlock) void composite_kernel(CompositeArgs A,
+__global__ __launch_bounds__(kBlock, ORD == ORD_HESS ? 3 : 1) void composite_kernel(CompositeArgs A,
                                                            const double *__restrict__ xyz,
                                                            int64_t n, double *__restrict__ phi,
                                                            double *__restrict__ force,
