@@ -323,22 +323,24 @@ def bench_high_order(dev, n: int = 10_000_000):
             "achieved_tflops": ach, "frac_fp64_peak": ach / PEAK_FP64_TFLOPS, "opcount": info}
 
 
-def bench_grad(dev, reps: int = 20, warmup: int = 3):
+def bench_grad(dev, reps: int = 20, warmup: int = 10):
     """Force fit: Multipole lmax 8 (2 x 81 x 50 live table entries) shifted
     along a 200-knot spline centre, N=1M tracers; one step = loss (mean
-    squared force residual) forward + backward to the tables, knots and
+    squared force residual, sum over particles and components / N, torch's
+    fused squared-error sum) forward + backward to the tables, knots and
     positions."""
     import torch
     from agama_gpu_amd.workloads import fit_problem
     model, xyz, F_t = fit_problem(1_000_000, 200, device=dev)
     xyz = xyz.clone().requires_grad_(True)
     params = [model.multipole.phi_table, model.multipole.dphi_table, model.center.knots, xyz]
+    n = xyz.shape[0]
 
     def step():
         for q in params:
             q.grad = None
         F = model(xyz, 0.73)[1]
-        loss = ((F - F_t) ** 2).sum(1).mean()
+        loss = torch.nn.functional.mse_loss(F, F_t, reduction="sum") / n
         loss.backward()
         return loss
 
