@@ -141,6 +141,32 @@ __device__ __forceinline__ double agx_log(double x)
     return fma(de, sconst(6.93147180369123816490e-01), fma(de, sconst(1.90821492927058770002e-10), lnm));
 }
 
+// ln x for x > 0 given also xm1 = x - 1 to full relative precision: where x
+// lies in [1/sqrt 2, sqrt 2) the series variable (x - 1) / (x + 1) is formed
+// from xm1, so an argument near 1 keeps the relative precision of its
+// distance from 1 (agx_log(x) would have only that of x itself there).
+__device__ __forceinline__ double agx_log_xm1(double x, double xm1)
+{
+    double m = __builtin_amdgcn_frexp_mant(x);     // [0.5, 1)
+    int e = __builtin_amdgcn_frexp_exp(x);
+    if (m < 0.70710678118654752440) { m += m; --e; }
+    const double f = (e == 0 ? xm1 : m - 1.0) * rcp_nr(m + 1.0);
+    const double f2 = f * f;
+    double p = sconst(1.0 / 21.0);
+    p = fma(p, f2, sconst(1.0 / 19.0));
+    p = fma(p, f2, sconst(1.0 / 17.0));
+    p = fma(p, f2, sconst(1.0 / 15.0));
+    p = fma(p, f2, sconst(1.0 / 13.0));
+    p = fma(p, f2, sconst(1.0 / 11.0));
+    p = fma(p, f2, sconst(1.0 / 9.0));
+    p = fma(p, f2, sconst(1.0 / 7.0));
+    p = fma(p, f2, sconst(1.0 / 5.0));
+    p = fma(p, f2, sconst(1.0 / 3.0));
+    const double lnm = fma(2.0 * f * f2, p, 2.0 * f);
+    const double de = (double)e;
+    return fma(de, sconst(6.93147180369123816490e-01), fma(de, sconst(1.90821492927058770002e-10), lnm));
+}
+
 // e^x for |x| < 700, within a few ulp: x = k ln2 + r with |r| <= ln2/2
 // (Cody-Waite reduction, ln 2 in two parts), e^r by its Taylor series to r^13
 // (the next term is below 2^-57), scaled by 2^k.  About 20 VALU instructions.

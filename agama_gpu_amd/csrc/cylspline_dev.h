@@ -27,14 +27,19 @@ struct CSK {
     const double *W;              // [(lmax_o+1)^2] outer PowerLaw, index l(l+1)+m
     int nR, nz, n_harm, log_scaling, lmax_o, mmax_o;
     double Rscale, r0;
-    double lnRs;                  // ln Rscale
+    double iRs;                   // 1 / Rscale
 };
 
 // The point's cylindrical radius and scaled coordinates.  asinh(X / Rscale)
-// = ln(X + sqrt(X^2 + Rscale^2)) - ln Rscale, whose derivative in X,
-// 1/sqrt(X^2 + Rscale^2), comes from the same square root; lR and |lz| are
-// clamped at 0 (rounding near the axis and the midplane must not move a point
-// below a grid that starts at 0).  R^2 below 2^-767 is the axis: R = 1/R = 0.
+// = ln a, a = (X + q) / Rscale with q = sqrt(X^2 + Rscale^2), whose derivative
+// in X, 1/q, comes from the same square root.  The logarithm also takes
+// a - 1 = (X + X^2 / (q + Rscale)) / Rscale, formed without the cancellation
+// of X + q - Rscale, so that a coordinate much smaller than Rscale keeps its
+// relative precision (ln(X + q) - ln Rscale loses it: near the axis the
+// Hessian's Phi_R / R - Phi_RR would then carry errors of 1e-7 relative to
+// Phi_RR at R = 1e-9 Rscale).  lR and |lz| are clamped at 0 (rounding near
+// the axis and the midplane must not move a point below a grid that starts at
+// 0).  R^2 below 2^-767 is the axis: R = 1/R = 0.
 struct CylCoords {
     double R, iR, lR, lz, jR, jz;
 };
@@ -51,7 +56,11 @@ __device__ __forceinline__ CylCoords cyl_coords(const CSK &P, double x, double y
     sqrt_rsqrt(fma(z, z, Rs2), qz, c.jz);
     // (compared, not fmax: a NaN coordinate stays NaN, fails every bounds test
     // of the caller and gives a NaN result, as in the reference)
-    const double tR = agx_log(c.R + qR) - P.lnRs, tz = agx_log(fabs(z) + qz) - P.lnRs;
+    const double az = fabs(z);
+    const double tR = agx_log_xm1((c.R + qR) * P.iRs,
+                                  fma(c.R, c.R * rcp_nr(qR + P.Rscale), c.R) * P.iRs);
+    const double tz = agx_log_xm1((az + qz) * P.iRs,
+                                  fma(az, az * rcp_nr(qz + P.Rscale), az) * P.iRs);
     c.lR = tR < 0.0 ? 0.0 : tR;
     c.lz = copysign(tz < 0.0 ? 0.0 : tz, z);
     return c;
@@ -95,20 +104,25 @@ __device__ __forceinline__ void stage_grids(const CSK &P, double *grids)
 // (the reference's evalCubic1D, :110-128, in weight form): h = x1 - x0,
 // ih = 1/h, t = (x - x0) ih, T = 1 - t.  A quantity with values c0, c1 and
 // slopes s0, s1 at the cell's two nodes is
-//     c1 + (c0 - c1) w + s0 u0 + s1 u1,
-//     w = T^2 (1 + 2t),  u0 = h t T^2,  u1 = -h t^2 T
-// (the upper node's value weight is 1 - w), and its first and second
-// derivatives in x take dw, du0, du1 and d2w, d2u0, d2u1.  The weights depend
-// on the point only, so every harmonic of a cell shares them: per harmonic
-// the bicubic patch costs 10 FMA per quantity and axis instead of forming
-// the one-dimensional Hermite polynomial afresh for each.
+//     c0 w0 + c1 w1 + s0 u0 + s1 u1,
+//     w0 = T^2 (1 + 2t),  w1 = t^2 (1 + 2T),  u0 = h t T^2,  u1 = -h t^2 T,
+// and its first and second derivatives in x are (c0 - c1) dw + s0 du0 + s1 du1
+// and the same with d2w, d2u0, d2u1 (the value weights' derivatives are
+// opposite).  The value keeps both weights rather than c1 + (c0 - c1) w0: near
+// a node whose value is 0 (an m != 0 harmonic on the axis, where the value
+// grows as R^2 and is then divided by R) w0 rounds to 1 and the difference
+// form would lose every digit of the value.  The weights depend on the point
+// only, so every harmonic of a cell shares them: per harmonic the bicubic
+// patch costs 10 FMA per quantity and axis instead of forming the
+// one-dimensional Hermite polynomial afresh for each.
 template <bool D1, bool D2>
 struct HermAxis {
-    double w, u0, u1, dw, du0, du1, d2w, d2u0, d2u1;
+    double w0, w1, u0, u1, dw, du0, du1, d2w, d2u0, d2u1;
     __device__ __forceinline__ HermAxis(double t, double h, double ih)
     {
         const double T = 1.0 - t, tT = t * T;
-        w = T * T * fma(2.0, t, 1.0);
+        w0 = T * T * fma(2.0, t, 1.0);
+        w1 = t * t * fma(2.0, T, 1.0);
         u0 = h * tT * T;
         u1 = -h * tT * t;
         if (D1) {
@@ -129,8 +143,8 @@ struct HermAxis {
                                          double &df, double &d2f) const
     {
         static_assert((!E1 || D1) && (!E2 || D2), "weights not formed");
+        f = fma(c0, w0, fma(c1, w1, fma(s0, u0, s1 * u1)));
         const double dif = c0 - c1;
-        f = fma(dif, w, fma(s0, u0, fma(s1, u1, c1)));
         if (E1) df = fma(dif, dw, fma(s0, du0, s1 * du1));
         if (E2) d2f = fma(dif, d2w, fma(s0, d2u0, s1 * d2u1));
     }
@@ -540,7 +554,7 @@ inline CSK make_cyl_args(const agx_cylspline *cs)
     P.nR = cs->nR; P.nz = cs->nz; P.n_harm = cs->n_harm; P.log_scaling = cs->log_scaling;
     P.lmax_o = cs->lmax_outer; P.mmax_o = cs->mmax_outer;
     P.Rscale = cs->Rscale; P.r0 = cs->r0_outer;
-    P.lnRs = std::log(cs->Rscale);
+    P.iRs = 1.0 / cs->Rscale;
     return P;
 }
 

@@ -97,3 +97,70 @@ def sweep_points(desc: dict, n: int, seed: int) -> np.ndarray:
     x[k:k + 5] = [[0, 0, desc["r_min"] * 3], [0, 0, -desc["r_max"] / 3], [0, 0, 0],
                   [desc["r_max"] * (1 - 1e-9), 0, 0], [0, desc["r_max"] * (1 + 1e-9), 0]]
     return x
+
+
+CYL_PATTERNS = ("even", "cos", "full", "subset")
+
+
+def random_cylspline(seed: int):
+    """(CylSplineCoefs, description) of a random CylSpline table: mmax 0..6
+    with even, cos-only, all or a random subset of harmonics, 6..32 R nodes
+    (R = 0 first, then geometric), and either a z >= 0 half grid (mirrored by
+    the build) or a full grid with a z-asymmetric field.  The m = 0 harmonic
+    is a Miyamoto-Nagai potential, the others eps_m (-GM) R^|m| /
+    (R^2 + z^2 + c^2)^((|m|+1)/2) (regular on the axis), times
+    (1 + k z / sqrt(z^2 + c^2)) on full grids."""
+    from agama_gpu_amd.coefs import CylSplineCoefs
+    rng = np.random.default_rng(5000 + seed)
+    mmax = int(rng.choice([0, 1, 2, 3, 4, 6]))
+    pattern = CYL_PATTERNS[seed % len(CYL_PATTERNS)]
+    half = bool(seed % 2 == 0)
+    nR = int(rng.integers(6, 33))
+    nzh = int(rng.integers(5, 25))
+    R_max = 10.0 ** rng.uniform(0.5, 2.5)
+    z_max = R_max * 10.0 ** rng.uniform(-0.5, 0.5)
+    R = np.concatenate([[0.0], R_max * np.geomspace(10.0 ** -rng.uniform(1.5, 3.0), 1.0, nR - 1)])
+    zp = z_max * np.geomspace(10.0 ** -rng.uniform(1.5, 3.0), 1.0, nzh)
+    z = np.concatenate([[0.0], zp]) if half else np.concatenate([-zp[::-1], [0.0], zp])
+    GM = 4.30091727067736e-06 * 10.0 ** rng.uniform(9.5, 11.5)
+    a, b, c = (R_max * 10.0 ** rng.uniform(-2.0, -0.5) for _ in range(3))
+    kz = 0.0 if half else rng.uniform(-0.5, 0.5)
+    RR, ZZ = np.meshgrid(R, z, indexing="ij")
+    ms = [m for m in range(-mmax, mmax + 1)]
+    if pattern == "even":
+        ms = [m for m in ms if m % 2 == 0]
+    elif pattern == "cos":
+        ms = [m for m in ms if m >= 0]
+    elif pattern == "subset":
+        ms = [m for m in ms if m == 0 or rng.random() < 0.5]
+    phi = {}
+    for m in ms:
+        if m == 0:
+            t = -GM / np.sqrt(RR ** 2 + (a + np.sqrt(ZZ ** 2 + b ** 2)) ** 2)
+        else:
+            am = abs(m)
+            eps = np.clip(rng.normal(), -2.0, 2.0) * 0.1 * 0.7 ** am
+            t = eps * -GM * RR ** am / (RR ** 2 + ZZ ** 2 + c ** 2) ** ((am + 1) / 2)
+            t = t * (1.0 + kz * ZZ / np.sqrt(ZZ ** 2 + c ** 2))
+        phi[m] = np.asarray(t, float)
+    coefs = CylSplineCoefs(R, z, phi, mmax)
+    desc = dict(seed=seed, mmax=mmax, pattern=pattern, half=half, nR=nR, nz=len(z),
+                R_max=float(R_max), z_max=float(z_max), ms=ms)
+    return coefs, desc
+
+
+def cyl_sweep_points(desc: dict, n: int, seed: int) -> np.ndarray:
+    """Radii log-uniform over [R_max / 1000, 5 max(R_max, z_max)], isotropic
+    directions (inside and outside the grid box); 2 % within 1e-9 R_max of
+    the z axis, a few on it and in the midplane."""
+    rng = np.random.default_rng(seed)
+    top = 5.0 * max(desc["R_max"], desc["z_max"])
+    r = np.exp(rng.uniform(np.log(desc["R_max"] / 1000.0), np.log(top), n))
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    x = u * r[:, None]
+    k = max(1, n // 50)
+    x[:k, :2] *= (1e-9 * desc["R_max"] / np.maximum(np.hypot(x[:k, 0], x[:k, 1]), 1e-300))[:, None]
+    x[k:k + 3] = [[0.0, 0.0, 0.3 * desc["z_max"]], [0.0, 0.0, -0.6 * desc["z_max"]],
+                  [0.4 * desc["R_max"], 0.2 * desc["R_max"], 0.0]]
+    return x
