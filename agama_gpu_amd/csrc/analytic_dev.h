@@ -207,7 +207,24 @@ __device__ __forceinline__ void loghalo(const double *p, double x, double y, dou
 //   H = h/2 (exp(-|z|/h) - 1 + |z|/h)  (h > 0, exponential),
 //       |h|/2 ln cosh(z/|h|)            (h < 0, isothermal),
 //       |z|/2                           (h = 0, thin)
-// (reference :950-1052 uses the exponential H for every sign of h)
+// (reference :950-1052 uses the exponential H for every sign of h).  Near
+// the midplane H ~ z^2: both profiles and their slopes are formed without the
+// cancellation of the terms of order 1 (the reference's exp(-u) - 1 + u keeps
+// only 1e-16 / u^2 of relative precision), from expm1 and a series.
+__device__ __forceinline__ double exp_m1_plus(double u)     // e^-u - 1 + u, u >= 0
+{
+    if (u >= 0.5) return u + expm1(-u);
+    // u^2 sum_k (-u)^k / (k+2)!, to k = 14 (0.5^15 / 17! ~ 1e-19)
+    double p = 1.0 / 355687428096000.0;                    // 1/16!
+    const double c[14] = {1.0 / 20922789888000.0, 1.0 / 1307674368000.0, 1.0 / 87178291200.0,
+                          1.0 / 6227020800.0, 1.0 / 479001600.0, 1.0 / 39916800.0,
+                          1.0 / 3628800.0, 1.0 / 362880.0, 1.0 / 40320.0, 1.0 / 5040.0,
+                          1.0 / 720.0, 1.0 / 120.0, 1.0 / 24.0, 1.0 / 6.0};
+#pragma unroll
+    for (int k = 0; k < 14; ++k) p = fma(-u, p, c[k]);
+    return u * u * fma(-u, p, 0.5);
+}
+
 template <bool G, bool H, bool RHO>
 __device__ __forceinline__ void disk_ansatz(const double *p, double x, double y, double z, Eval3 &o)
 {
@@ -219,13 +236,19 @@ __device__ __forceinline__ void disk_ansatz(const double *p, double x, double y,
     double Hv, H1, H2;
     const double az = fabs(z), sg = (z > 0.0) ? 1.0 : ((z < 0.0) ? -1.0 : 0.0);
     if (hz > 0.0) {
-        const double u = az / hz, eu = exp(-u);
-        Hv = 0.5 * hz * (eu - 1.0 + u); H1 = 0.5 * sg * (1.0 - eu); H2 = 0.5 * eu / hz;
+        const double u = az / hz, em = expm1(-u);            // e^-u - 1
+        Hv = 0.5 * hz * exp_m1_plus(u); H1 = -0.5 * sg * em; H2 = 0.5 * (1.0 + em) / hz;
     } else if (hz < 0.0) {
-        const double h = -hz, u = az / h, eu = exp(-2.0 * u);
-        // ln cosh(u) = u + ln(1 + e^{-2u}) - ln 2, stable for large u
-        Hv = 0.5 * h * (u + log1p(eu) - M_LN2);
-        const double th = (1.0 - eu) / (1.0 + eu);
+        const double h = -hz, u = az / h, em = expm1(-2.0 * u);
+        // ln cosh(u): ln(1 + 2 sinh^2(u/2)) near 0, u + ln(1 + e^{-2u}) - ln 2
+        // beyond (no overflow for large u)
+        if (u < 1.0) {
+            const double sh = 0.5 * (expm1(0.5 * u) - expm1(-0.5 * u));
+            Hv = 0.5 * h * log1p(2.0 * sh * sh);
+        } else {
+            Hv = 0.5 * h * (u + log1p(1.0 + em) - M_LN2);
+        }
+        const double th = -em / (2.0 + em);                  // tanh(u)
         H1 = 0.5 * sg * th; H2 = 0.5 * (1.0 - th * th) / h;
     } else {
         Hv = 0.5 * az; H1 = 0.5 * sg; H2 = 0.0;

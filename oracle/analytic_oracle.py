@@ -135,6 +135,17 @@ def loghalo(x, v0, rc, p, q):                          # :842-892
     return 0.5 * v02 * np.log(m2), g, h, lap * INV_4PIG
 
 
+def _exp_m1_plus_series(u):
+    """e^-u - 1 + u = sum_{k>=2} (-u)^k / k! summed in long double to k = 25."""
+    u = np.asarray(u, np.longdouble)
+    out = np.zeros_like(u)
+    term = u * u / 2
+    for k in range(2, 26):
+        out += term
+        term = term * -u / (k + 1)
+    return out.astype(np.float64)
+
+
 def disk_ansatz(x, Sigma, Rd, hz, rin=0.0):            # :950-1052
     X, Y, Z = x[:, 0], x[:, 1], x[:, 2]
     r = np.maximum(np.linalg.norm(x, axis=1), 1e-300)
@@ -144,13 +155,18 @@ def disk_ansatz(x, Sigma, Rd, hz, rin=0.0):            # :950-1052
     f1, f2 = f * fo, f * (fo * fo - 2.0 * rin * ir ** 3)
     az, sg = np.abs(Z), np.sign(Z)
     if hz > 0:
+        # e^-u - 1 + u and 1 - e^-u without the cancellation near the midplane
+        # (the reference's form keeps 1e-16 / u^2): a Taylor sum for u < 0.5
         u = az / hz
-        eu = np.exp(-u)
-        Hv, H1, H2 = 0.5 * hz * (eu - 1.0 + u), 0.5 * sg * (1.0 - eu), 0.5 * eu / hz
+        g = np.where(u >= 0.5, u + np.expm1(-u), _exp_m1_plus_series(u))
+        Hv, H1, H2 = 0.5 * hz * g, -0.5 * sg * np.expm1(-u), 0.5 * np.exp(-u) / hz
     elif hz < 0:                                      # Agama isothermal: |h|/2 ln cosh(z/|h|)
         h = -hz
         u = az / h
-        Hv = 0.5 * h * (u + np.log1p(np.exp(-2.0 * u)) - math.log(2.0))   # ln cosh without overflow
+        us = np.minimum(u, 1.0)                       # (the branch np.where discards)
+        lc = np.where(u < 1.0, np.log1p(2.0 * np.sinh(0.5 * us) ** 2),
+                      u + np.log1p(np.exp(-2.0 * np.maximum(u, 1.0))) - math.log(2.0))
+        Hv = 0.5 * h * lc                             # ln cosh without overflow or cancellation
         th = np.tanh(Z / h)
         H1, H2 = 0.5 * th, 0.5 * (1.0 - th * th) / h
     else:                                             # Agama thin disk: |z|/2

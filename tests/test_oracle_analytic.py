@@ -94,3 +94,32 @@ def test_transform_chain_rule():
                     A.transformed(f, x - step * e[k], 0.7, 1.3, (1.0, -2.0, 0.5))[0]) / (2 * step[:, 0])
                    for k in range(3)], axis=1)
     assert np.max(np.abs(fd - g)) <= 1e-6 * np.max(np.abs(g))
+
+
+@pytest.mark.parametrize("hz", [0.3, -0.3])
+def test_disk_ansatz_midplane_against_mpmath(hz):
+    """Near the midplane the vertical profiles grow as z^2; the oracle's
+    Phi and dPhi/dz there equal a 50-digit mpmath evaluation of the closed
+    forms h/2 (e^-u - 1 + u) and |h|/2 ln cosh u (u = |z|/|h|) to 1e-14
+    (the reference's form e^-u - 1 + u keeps only 1e-16 / u^2)."""
+    import mpmath as mp
+    mp.mp.dps = 50
+    Sigma, Rd = 5e8, 3.0
+    z = np.concatenate([-np.geomspace(1e-9, 2.0, 40), np.geomspace(1e-9, 2.0, 40)])
+    x = np.stack([np.full_like(z, 1.7), np.full_like(z, -0.4), z], axis=1)
+    P, g, _, _ = A.disk_ansatz(x, Sigma, Rd, hz)
+    for k in range(len(z)):
+        X, Y, Z = (mp.mpf(float(v)) for v in x[k])
+        r = mp.sqrt(X * X + Y * Y + Z * Z)
+        f = 4 * mp.pi * mp.mpf(A.G) * Sigma * mp.exp(-r / Rd)
+        h = abs(mp.mpf(hz))
+        u = abs(Z) / h
+        if hz > 0:
+            Hv, H1 = h / 2 * (mp.exp(-u) - 1 + u), mp.sign(Z) * (1 - mp.exp(-u)) / 2
+        else:
+            Hv, H1 = h / 2 * mp.log(mp.cosh(u)), mp.tanh(Z / h) / 2
+        fr = -f / Rd
+        ref_P = f * Hv
+        ref_gz = fr * Hv * Z / r + f * H1
+        assert abs(P[k] - float(ref_P)) <= 1e-14 * abs(float(ref_P)), (hz, z[k])
+        assert abs(g[k, 2] - float(ref_gz)) <= 1e-14 * abs(float(ref_gz)), (hz, z[k])
