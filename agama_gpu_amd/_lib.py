@@ -84,6 +84,7 @@ AGX_OPT_MP_BIN_GRID = 7
 AGX_OPT_GRID_PER_CU = 8
 AGX_OPT_SORT_GRID = 9
 AGX_OPT_CYL_PATH = 10
+AGX_OPT_ADJ_STAGE = 11
 AGX_BATCH_MAX_ITEM = 16384
 AGX_NOALLOC_MAX_N = 11_000_000
 AGX_E_SCRATCH = -8

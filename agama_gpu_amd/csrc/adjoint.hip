@@ -45,19 +45,92 @@ __device__ __forceinline__ void scatter6(double *g, double s, double a, double b
     }
 }
 
+// Staged table gradient (the SLAB kernels).  Per term the wavefront's 64
+// particles each contribute alpha s^j + j beta s^(j-1) to the six Horner
+// coefficients; instead of six cross-lane sums per term, (alpha, beta) of
+// kAdjBatch terms are staged in a wavefront-private LDS block and summed in
+// one pass in which lane group t (8 lanes) takes term t and each lane eight
+// particles: eight products per particle and coefficient pair, then three
+// 8-lane butterfly steps per coefficient, and one LDS add per coefficient and
+// term into the slab.  About 25 VALU instructions per term against 49 for
+// the per-term cross-lane sums.
+constexpr int kAdjBatch = 8;
+static int g_adj_stage = 1;      // AGX_OPT_ADJ_STAGE
+struct AdjStage {
+    dvec2 ab[kAdjBatch][64];     // (alpha, beta) of the staged terms, per lane
+    double s[64];                // each lane's position in its interval
+    int off[kAdjBatch];          // slab offsets of the staged terms
+};
+
+// Sum over the 8 lanes of each half-row (every lane gets its group's sum).
+__device__ __forceinline__ double sum8(double v)
+{
+    v += dpp_f64<0xB1>(v);    // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);    // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141>(v);   // row_half_mirror
+    return v;
+}
+
+// Adds the staged terms' coefficient sums to the slab g; every lane of the
+// wavefront takes part (the butterfly reads all of them).
+__device__ __forceinline__ void flush_stage(AdjStage *st, double *g, int count, int lane)
+{
+    __builtin_amdgcn_wave_barrier();
+    const int t = lane >> 3, q = lane & 7;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0, a5 = 0.0;
+    if (t < count) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int pt = q + 8 * u;
+            const dvec2 ab = st->ab[t][pt];
+            const double s = st->s[pt], s2 = s * s, s3 = s2 * s, s4 = s2 * s2;
+            const double b = ab.y;
+            a0 += ab.x;
+            a1 += fma(ab.x, s, b);
+            a2 += fma(ab.x, s2, 2.0 * b * s);
+            a3 += fma(ab.x, s3, 3.0 * b * s2);
+            a4 += fma(ab.x, s4, 4.0 * b * s3);
+            a5 += fma(ab.x, s4 * s, 5.0 * b * s4);
+        }
+    }
+    a0 = sum8(a0); a1 = sum8(a1); a2 = sum8(a2);
+    a3 = sum8(a3); a4 = sum8(a4); a5 = sum8(a5);
+    if (t < count && q == 0) {
+        double *gt = g + st->off[t];
+        gt[0] += a0; gt[1] += a1; gt[2] += a2; gt[3] += a3; gt[4] += a4; gt[5] += a5;
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
 // Table gradient of one step of the interior evaluation (the hook of
 // interior<ORD_HESS>): per cos/sin term, the gradient of the quintic's six
 // coefficients is s^j alpha + j s^(j-1) beta with
 //   alpha = lam (w_phi Q T + w_theta dQ T + w_phi' X) + w_lnr mu Q T,
 //   beta  = w_lnr lam / dlogr Q T,
 // X the term's d/dphi basis over sin(theta) (its pole limit for m = 1).
-template <bool SLAB>
+template <bool SLAB, bool STAGE>
 struct AdjHook {
     double *gslab;
     double *sum;                 // [4]: l > 0 sums of value, d/ds, d/dtheta, d/dphi
     double s, lam, mu, idl, wphi, wPr, wPt, wPf;
     double psg;                  // sign of cos(theta): the pole limit's sign
     int lane;
+    AdjStage *st;                // STAGE: the wavefront's staging block
+    mutable int staged;          // STAGE: terms staged in st (wavefront-uniform)
+    // (alpha, beta) of a term at slab offset off: staged (STAGE) or summed now
+    __device__ void add(int off, double alpha, double beta) const
+    {
+        if constexpr (STAGE) {
+            st->ab[staged][lane] = dvec2{alpha, beta};
+            if (lane == 0) st->off[staged] = off;
+            if (++staged == kAdjBatch) {
+                flush_stage(st, gslab, kAdjBatch, lane);
+                staged = 0;
+            }
+        } else {
+            scatter6<SLAB>(gslab + off, s, alpha, beta, lane);
+        }
+    }
     __device__ void side(int off, double Q, double dQ, int m, double T, double dT, double dTp,
                          double ist, bool pole) const
     {
@@ -66,7 +139,7 @@ struct AdjHook {
         const double QT = Q * T;
         const double alpha = lam * (wphi * QT + wPt * dQ * T + wPf * X) + wPr * mu * QT;
         const double beta = wPr * lam * idl * QT;
-        scatter6<SLAB>(gslab + off, s, alpha, beta, lane);
+        add(off, alpha, beta);
     }
     __device__ void term(int oc, int os, double Q, double dQ, int m, double cm, double sm,
                          double ist, bool pole) const
@@ -85,11 +158,11 @@ struct AdjHook {
 // (LDS slab or memory); the invPhi0 gradient is accumulated per lane in ginv.
 // One pass: the Hessian-vector product's evaluation carries the table
 // gradient's scatter as its per-term hook.
-template <bool SLAB, int STRIDE>
+template <bool SLAB, int STRIDE, bool STAGE>
 __device__ __forceinline__ void adjoint_interior(const MPK &P, int k, bool valid, double x, double y,
                                                  double z, double wphi, const double wf[3],
                                                  double *gx, double *gslab, double &ginv_acc,
-                                                 int lane)
+                                                 int lane, AdjStage *stg)
 {
     const double *slab = P.poly + (int64_t)k * P.n_lm * 6;
     if (!valid) {   // a finite point inside interval k; its seeds are zero
@@ -130,10 +203,12 @@ __device__ __forceinline__ void adjoint_interior(const MPK &P, int k, bool valid
     // evaluation with the Hessian (for dL/dx) and the per-term scatter
     double sum[5];
     const double psg = ct < 0.0 ? -1.0 : 1.0;
-    const AdjHook<SLAB> hook{gslab, sum, s, lam, mu, idl, wphi, wPr, wPt, wPf, psg, lane};
+    if (STAGE) stg->s[lane] = s;
+    const AdjHook<SLAB, STAGE> hook{gslab, sum, s, lam, mu, idl, wphi, wPr, wPt, wPf, psg, lane,
+                                    stg, 0};
     {
         PointOut o{};
-        interior<ORD_HESS, true, AdjHook<SLAB>, STRIDE>(P, slab, x, y, z, r2, r, ir, lnr, k, o,
+        interior<ORD_HESS, true, AdjHook<SLAB, STAGE>, STRIDE>(P, slab, x, y, z, r2, r, ir, lnr, k, o,
                                                           0.0, hook);
         if (valid) {
             gx[0] = wphi * o.g[0] - (o.h[0] * wfx + o.h[3] * wfy + o.h[5] * wfz);
@@ -155,7 +230,8 @@ __device__ __forceinline__ void adjoint_interior(const MPK &P, int k, bool valid
         gc0 = wphi;
         gc0d = wPr * idl;
     }
-    scatter6<SLAB>(gslab, s, gc0, gc0d, lane);
+    hook.add(0, gc0, gc0d);
+    if (STAGE && hook.staged > 0) flush_stage(stg, gslab, hook.staged, lane);
     ginv_acc += ginv;
 }
 
@@ -216,7 +292,7 @@ __device__ __forceinline__ void flush_slab(double *slab, double *g, int cnt, int
 // STRIDE 1: the program recurs by 1 only (AGX_PROG_STRIDE1); 0: any program.
 // GSUM: also add minus the sum over particles of dL/dx to gscal[5..7] (the
 // gradient with respect to a centre subtracted from every position).
-template <bool SLAB, int STRIDE, bool GSUM>
+template <bool SLAB, int STRIDE, bool GSUM, bool STAGE>
 __global__ __launch_bounds__(kAdjBlock, 3) void mp_adjoint_kernel(
     MPK P, const double *__restrict__ xyz, const double *__restrict__ gphi,
     const double *__restrict__ gforce, double *__restrict__ gxyz, double *__restrict__ gpoly,
@@ -227,6 +303,8 @@ __global__ __launch_bounds__(kAdjBlock, 3) void mp_adjoint_kernel(
     const int lane = threadIdx.x & 63;
     const int slab_n = P.n_lm * 6;
     double *slab = adj_lds + (SLAB ? (threadIdx.x >> 6) * slab_n : 0);
+    // STAGE: the wavefronts' staging blocks follow their slabs
+    AdjStage *st = (AdjStage *)(adj_lds + (kAdjBlock / 64) * slab_n) + (threadIdx.x >> 6);
     if (SLAB)
         for (int q = lane; q < slab_n; q += 64) slab[q] = 0.0;
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};   // invPhi0, inner U, W, outer U, W
@@ -249,9 +327,9 @@ __global__ __launch_bounds__(kAdjBlock, 3) void mp_adjoint_kernel(
         }
         double gx[3] = {0.0, 0.0, 0.0};
         if (b > 0)
-            adjoint_interior<SLAB, STRIDE>(P, b - 1, valid, x, y, z, wphi, wf, gx,
+            adjoint_interior<SLAB, STRIDE, STAGE>(P, b - 1, valid, x, y, z, wphi, wf, gx,
                                             SLAB ? slab : gpoly + (int64_t)(b - 1) * slab_n,
-                                            acc[0], lane);
+                                            acc[0], lane, st);
         else
             adjoint_extrap(P, valid, x, y, z, wphi, wf, gx, acc + 1);
         if (valid && gxyz) { gxyz[3 * i] = gx[0]; gxyz[3 * i + 1] = gx[1]; gxyz[3 * i + 2] = gx[2]; }
@@ -273,6 +351,14 @@ __global__ __launch_bounds__(kAdjBlock, 3) void mp_adjoint_kernel(
         __builtin_amdgcn_wave_barrier();
         if (lane < 3 && gsum[lane] != 0.0) atomicAdd(gscal + 5 + lane, -gsum[lane]);
     }
+}
+
+// The adjoint's option of agx_set_option (which lives in multipole.hip).
+int adj_set_stage(int64_t value)
+{
+    if (value < 0 || value > 1) return AGX_E_SIZE;
+    g_adj_stage = (int)value;
+    return AGX_OK;
 }
 
 }  // namespace agx
@@ -300,16 +386,22 @@ extern "C" int agx_multipole_adjoint(const agx_multipole *mp, const double *d_xy
     hipStream_t st = (hipStream_t)stream;
     if (!(flags & AGX_WORK_SORTED)) sort_launch(MpBinKey{P, d_xyz}, n, nb, W, st);
     const size_t slab_bytes = (size_t)mp->n_lm * 6 * sizeof(double) * (kAdjBlock / 64);
+    const size_t stage_bytes = slab_bytes + sizeof(AdjStage) * (kAdjBlock / 64);
     const bool slab = slab_bytes <= (size_t)kAdjSlabLds;
+    const bool stage = g_adj_stage && stage_bytes <= (size_t)kAdjSlabLds;
     const bool s1 = mp->prog_flags & AGX_PROG_STRIDE1;
     const bool gs = flags & AGX_ADJ_CENTER_GRAD;
     auto pick = [&](auto gsum) {
         constexpr bool G = decltype(gsum)::value;
-        return slab ? (s1 ? mp_adjoint_kernel<true, 1, G> : mp_adjoint_kernel<true, 0, G>)
-                    : (s1 ? mp_adjoint_kernel<false, 1, G> : mp_adjoint_kernel<false, 0, G>);
+        if (stage)
+            return s1 ? mp_adjoint_kernel<true, 1, G, true> : mp_adjoint_kernel<true, 0, G, true>;
+        return slab ? (s1 ? mp_adjoint_kernel<true, 1, G, false>
+                          : mp_adjoint_kernel<true, 0, G, false>)
+                    : (s1 ? mp_adjoint_kernel<false, 1, G, false>
+                          : mp_adjoint_kernel<false, 0, G, false>);
     };
     auto kern = gs ? pick(std::true_type{}) : pick(std::false_type{});
-    const size_t lds = slab ? slab_bytes : 0;
+    const size_t lds = stage ? stage_bytes : (slab ? slab_bytes : 0);
     const SortedShape sh = sorted_shape(
         n, nb, resident_workgroups((const void *)kern, kAdjBlock, lds, 256 * 3), kAdjBlock);
     // a reused sort has had its queue consumed; only the dynamic traversal reads it

@@ -343,6 +343,7 @@ constexpr double G_AGAMA_D = 4.30091727067736e-06;
 extern int g_grid_per_cu;
 // AGX_OPT_CYL_PATH of agx_set_option (defined in cylspline.hip).
 int cyl_set_path(int64_t value);
+int adj_set_stage(int64_t value);
 int cyl_path();
 
 // Workgroups of `kernel` (block threads, lds bytes of dynamic LDS each) that

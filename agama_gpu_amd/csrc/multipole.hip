@@ -341,6 +341,8 @@ extern "C" int agx_set_option(int key, int64_t value)
         return AGX_OK;
     case AGX_OPT_CYL_PATH:
         return cyl_set_path(value);
+    case AGX_OPT_ADJ_STAGE:
+        return adj_set_stage(value);
     default:
         return AGX_E_WHAT;
     }

@@ -434,6 +434,11 @@ int agx_leapfrog(const agx_component *d_leaves, int32_t n_leaves,
                                  from a copy in LDS (tables with nR + nz <= 512;
                                  larger ones take path 1), 1 one point per
                                  lane whichever branch it takes */
+#define AGX_OPT_ADJ_STAGE 11  /* Multipole adjoint's table-gradient sums: 1
+                                 (default) terms staged in LDS eight at a time
+                                 and summed in one pass where the staging
+                                 blocks and the slabs fit in 53 KiB per
+                                 workgroup, 0 six cross-lane sums per term */
 int agx_set_option(int key, int64_t value);
 
 /* Library identification and error text.  agx_version: major * 10000 +

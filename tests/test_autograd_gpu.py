@@ -303,3 +303,30 @@ def test_fused_shift_matches_explicit_shift(n):
     for a, b in zip(fused, explicit):
         a, b = a.cpu().numpy(), b.cpu().numpy()
         assert np.max(np.abs(a - b)) <= 1e-11 * np.abs(b).max()
+
+
+def test_adjoint_staged_sums_equal_cross_lane_sums():
+    """The two table-gradient reductions of the adjoint (terms staged in LDS
+    eight at a time, the default; six cross-lane sums per term,
+    AGX_OPT_ADJ_STAGE 0) give the same gradients to round-off, for a
+    stride-1 table with sin terms (FIRE) and the stride-2 halo table."""
+    from agama_gpu_amd import _lib
+    from agama_gpu_amd.autograd import FittableMultipole
+    for name in ("fire_dense", "halo_lmax8"):
+        c = _coefs(name)
+        x = _points(c, n=50_000, seed=9)
+        m = FittableMultipole(c, device=DEV)
+        rng = np.random.default_rng(9)
+        xyz = torch.as_tensor(x, device=DEV)
+        wphi = torch.as_tensor(rng.normal(size=len(x)), device=DEV)
+        wF = torch.as_tensor(rng.normal(size=(len(x), 3)), device=DEV)
+        out = {}
+        try:
+            for stage in (0, 1):
+                _lib.set_option(_lib.AGX_OPT_ADJ_STAGE, stage)
+                out[stage] = torch.autograd.grad(_loss(m, xyz, wphi, wF),
+                                                 [m.phi_table, m.dphi_table])
+        finally:
+            _lib.set_option(_lib.AGX_OPT_ADJ_STAGE, 1)
+        for a, b in zip(out[0], out[1]):
+            assert (torch.max(torch.abs(a - b)) / torch.max(torch.abs(a))).item() < 1e-12, name
