@@ -442,12 +442,15 @@ int agx_leapfrog(const agx_component *d_leaves, int32_t n_leaves,
 int agx_set_option(int key, int64_t value);
 
 /* Library identification and error text.  agx_version: major * 10000 +
- * minor * 100 + patch (10300: this header).  1.2.0: agx_component.prog_flags
+ * minor * 100 + patch (10400: this header).  1.2.0: agx_component.prog_flags
  * (formerly padding) is read; a position with a NaN coordinate gives NaN
  * outputs, and one whose r^2 overflows (|x| above about 1.3e154) gives the
  * outer-law limit 0 for Multipole tables.  1.3.0: agx_multipole.origin (a new
  * last field: callers built against 1.2 pass a shorter struct and must be
- * rebuilt) and agx_multipole_adjoint's AGX_ADJ_CENTER_GRAD flag. */
+ * rebuilt) and agx_multipole_adjoint's AGX_ADJ_CENTER_GRAD flag.  1.4.0: the
+ * AGX_OPT_ADJ_STAGE option; CylSpline coordinates and values near the z axis
+ * and the midplane, and DiskAnsatz profiles near the midplane, to full
+ * relative precision (results change at the rounding level elsewhere). */
 int agx_version(void);
 const char *agx_error_string(int code);
 

@@ -35,7 +35,7 @@ def test_library_exports_all_declared_symbols():
 def test_version_and_error_strings():
     from agama_gpu_amd import _lib
     L = _lib.load()
-    assert L.agx_version() == 10300
+    assert L.agx_version() == 10400
     assert b"NULL" in L.agx_error_string(-1)
     assert L.agx_error_string(12345) == b"unknown error code"
 
