@@ -78,6 +78,7 @@ AGX_ADJ_CENTER_GRAD = 2
 AGX_PROG_STRIDE1 = 1
 AGX_PROG_COS_ONLY = 2
 AGX_PROG_STRIDE2 = 4
+AGX_PROG_TRIAXIAL_SHIFT, AGX_PROG_TRIAXIAL_MASK = 8, 0xFF00
 AGX_OPT_MP_PATH, AGX_OPT_MP_TILE, AGX_OPT_MP_BIN_MIN_N, AGX_OPT_MP_BIN_MIN_TERMS = 1, 2, 3, 4
 AGX_OPT_MP_LDS_GRID = 6
 AGX_OPT_MP_BIN_GRID = 7
@@ -85,6 +86,7 @@ AGX_OPT_GRID_PER_CU = 8
 AGX_OPT_SORT_GRID = 9
 AGX_OPT_CYL_PATH = 10
 AGX_OPT_ADJ_STAGE = 11
+AGX_OPT_MP_TRIAXIAL = 12
 AGX_BATCH_MAX_ITEM = 16384
 AGX_NOALLOC_MAX_N = 11_000_000
 AGX_E_SCRATCH = -8

@@ -75,6 +75,8 @@ class MultipoleEnsembleGPU:
         self._prog_flags = 0xFFFFFFFF
         for p in self.potentials:
             self._prog_flags &= int(p.descriptor.prog_flags)
+        # (the ensemble kernel has no unrolled triaxial variants)
+        self._prog_flags &= ~_lib.AGX_PROG_TRIAXIAL_MASK & 0xFFFFFFFF
         path = np.array([c[0] for c in self.choices], np.int32)
         chunk = np.array([c[1] for c in self.choices], np.int32)
         cost = np.array([_cost_per_particle(p) for p in self.potentials], np.float64)

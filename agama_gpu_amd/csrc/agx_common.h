@@ -11,6 +11,7 @@ namespace agx {
 typedef __attribute__((address_space(4))) const double cdouble_t;
 typedef double dvec2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(4))) const dvec2 cdvec2_t;
+typedef __attribute__((address_space(3))) const dvec2 ldvec2_t;   // in LDS
 typedef __attribute__((address_space(4))) const int cint_t;
 
 __device__ __forceinline__ double cld(const double *p, int i)
@@ -116,6 +117,17 @@ __device__ __forceinline__ double vopaque(double v)
 {
     asm volatile("" : "+v"(v));
     return v;
+}
+
+// A pointer (wave-uniform, in scalar registers) that the compiler cannot
+// identify with its value before this point: loads through it at constant
+// offsets are not hoisted out of an enclosing loop, where their results
+// would be held in registers for the whole loop.
+template <class T>
+__device__ __forceinline__ const T *sopaque(const T *p)
+{
+    asm volatile("" : "+s"(p));
+    return p;
 }
 
 __device__ __forceinline__ double agx_log(double x)

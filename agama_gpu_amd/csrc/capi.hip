@@ -47,7 +47,7 @@ int64_t resident_workgroups(const void *kernel, int block, size_t lds, int64_t f
 // 1.3.0: agx_multipole.origin (new last field), AGX_ADJ_CENTER_GRAD
 // 1.4.0: AGX_OPT_ADJ_STAGE; CylSpline near-axis / midplane and DiskAnsatz
 // midplane precision
-extern "C" int agx_version(void) { return 10400; }
+extern "C" int agx_version(void) { return 10500; }
 
 extern "C" const char *agx_error_string(int code)
 {

@@ -1,6 +1,7 @@
 // Multipole kernels and C-ABI entry points (gfx950, FP64).
 #include <mutex>
 #include "multipole_dev.h"
+#include "multipole_lds.h"
 #include "sort_dev.h"
 
 namespace agx {
@@ -50,47 +51,6 @@ __global__ __launch_bounds__(kBinThreads) void mp_binned_kernel(
         }
         store<ORD>(o, i, phi, force, deriv);
     });
-}
-
-// One particle per lane with the whole coefficient table staged in LDS once
-// per workgroup: the per-term gathers of the lane kernel become LDS reads.
-// Used for tables of up to 160 KB (the LDS of one CU); the workgroup size is
-// chosen so that the workgroups that fit in LDS hold 16 wavefronts per CU.
-// STRIDE and SIN as for interior(): the program flags the variant requires.
-template <int ORD, int BLOCK, int STRIDE, bool SIN>
-__global__ __launch_bounds__(BLOCK) void mp_lds_kernel(MPK P, const double *__restrict__ xyz,
-                                                        int64_t n, double *__restrict__ phi,
-                                                        double *__restrict__ force,
-                                                        double *__restrict__ deriv, double inv4piG)
-{
-    P = with_law(P);
-    extern __shared__ __align__(16) double lds_tab[];
-    // Positions are read one grid-stride iteration ahead (the first before
-    // the table staging), so the HBM latency of the next particle overlaps
-    // the arithmetic of the current one.
-    const int64_t stride = (int64_t)gridDim.x * BLOCK;
-    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    double xn = 0.0, yn = 0.0, zn = 0.0;
-    if (i < n) { xn = xyz[3 * i]; yn = xyz[3 * i + 1]; zn = xyz[3 * i + 2]; }
-    const int nv = P.n_int * P.n_lm * 3;               // 16-byte words
-    const dvec2 *src = (const dvec2 *)P.poly;
-    dvec2 *dst = (dvec2 *)lds_tab;
-    for (int j = threadIdx.x; j < nv; j += BLOCK) dst[j] = src[j];
-    __syncthreads();
-    const int slab_len = P.n_lm * 6;
-    for (; i < n; i += stride) {
-        const double x = xn - P.ox, y = yn - P.oy, z = zn - P.oz;
-        const int64_t inx = i + stride;
-        if (inx < n) { xn = xyz[3 * inx]; yn = xyz[3 * inx + 1]; zn = xyz[3 * inx + 2]; }
-        PointOut o{};
-        Where w = classify(P, x, y, z);
-        if (w.kind == 2)
-            interior<ORD, false, NoHook, STRIDE, SIN>(P, lds_tab + w.k * slab_len, x, y, z,
-                                                       w.r2, w.r, w.ir, w.lnr, w.k, o, inv4piG);
-        else
-            extrap<ORD>(P, w, x, y, z, o, inv4piG);
-        store<ORD>(o, i, phi, force, deriv);
-    }
 }
 
 // The interval-sorted order (sort_dev.h): 64 particles of one interval per
@@ -156,6 +116,12 @@ static int check_mp(const agx_multipole *mp, int64_t n)
     if (mp->n_int < 1 || mp->n_lm < 1 || mp->n_groups < 0 || mp->n_steps < 0) return AGX_E_SIZE;
     if (mp->lmax > 32) return AGX_E_LMAX;
     if (mp->prog_flags & ~kProgFlags) return AGX_E_WHAT;
+    // a triaxial degree must agree with the program's size (its column
+    // offsets then stay inside the table's slabs)
+    const int tl = (mp->prog_flags & AGX_PROG_TRIAXIAL_MASK) >> AGX_PROG_TRIAXIAL_SHIFT;
+    if (tl && ((tl & 1) || mp->n_groups != tl / 2 + 1 || mp->n_steps != tri_step0(tl, tl / 2 + 1) ||
+               mp->n_lm != mp->n_steps))
+        return AGX_E_WHAT;
     return AGX_OK;
 }
 
@@ -166,6 +132,7 @@ static int64_t g_mp_bin_min_n = 200000;
 static int64_t g_mp_bin_min_terms = 24;
 static int g_mp_lds_grid = 4;        // LDS path: grid = this x resident workgroups
 static int g_mp_bin_grid = 2;        // binned path: grid = this x resident workgroups
+static int g_mp_triaxial = 1;        // LDS path: unrolled kernels for triaxial programs
 
 struct BinLaunch {
     unsigned grid;
@@ -197,12 +164,10 @@ static bool binned_auto(const agx_multipole *mp, int64_t n)
     return mp->n_int + 1 <= kMaxBins && n >= g_mp_bin_min_n && mp->n_lm >= g_mp_bin_min_terms;
 }
 
-constexpr int64_t kLdsPerCu = 160 * 1024;
-constexpr int64_t kLdsTableBytes = kLdsPerCu;
-
+// LDS bytes of the staged table (slabs padded: lds_slab_words)
 static int64_t table_bytes(const agx_multipole *mp)
 {
-    return (int64_t)mp->n_int * mp->n_lm * 6 * (int64_t)sizeof(double);
+    return (int64_t)mp->n_int * lds_slab_words(mp->n_lm) * 16;
 }
 
 static unsigned lds_grid(int64_t n, int block, int64_t per_cu)
@@ -213,28 +178,6 @@ static unsigned lds_grid(int64_t n, int block, int64_t per_cu)
     return (unsigned)want;
 }
 
-// Dynamic LDS beyond 64 KB has to be allowed per kernel.
-template <int ORD, int BLOCK, int STRIDE, bool SIN>
-static void lds_attr(int64_t bytes)
-{
-    static int64_t set = 0;
-    if (bytes > set) {
-        if (hipFuncSetAttribute((const void *)mp_lds_kernel<ORD, BLOCK, STRIDE, SIN>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsPerCu) ==
-            hipSuccess)
-            set = kLdsPerCu;
-    }
-}
-
-template <int ORD, int BLOCK, int STRIDE, bool SIN>
-static void launch_lds(const MPK &P, unsigned grid, int64_t tb, const double *xyz, int64_t n,
-                       double *phi, double *force, double *deriv, double inv4piG, hipStream_t s)
-{
-    if (BLOCK > 256) lds_attr<ORD, BLOCK, STRIDE, SIN>(tb);
-    hipLaunchKernelGGL((mp_lds_kernel<ORD, BLOCK, STRIDE, SIN>), dim3(grid), dim3(BLOCK),
-                       (size_t)tb, s, P, xyz, n, phi, force, deriv, inv4piG);
-}
-
 template <int ORD, int BLOCK>
 static void launch_lds(const agx_multipole *mp, const MPK &P, unsigned grid, int64_t tb,
                        const double *xyz, int64_t n, double *phi, double *force, double *deriv,
@@ -242,6 +185,10 @@ static void launch_lds(const agx_multipole *mp, const MPK &P, unsigned grid, int
 {
     const int st = prog_stride(mp->prog_flags);
     const bool co = mp->prog_flags & AGX_PROG_COS_ONLY;
+    const int tl = (mp->prog_flags & AGX_PROG_TRIAXIAL_MASK) >> AGX_PROG_TRIAXIAL_SHIFT;
+    if (tl && g_mp_triaxial &&
+        launch_lds_triaxial(ORD, BLOCK, tl, P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s))
+        return;
     if (st == 1 && co)
         launch_lds<ORD, BLOCK, 1, false>(P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s);
     else if (st == 1)
@@ -343,6 +290,10 @@ extern "C" int agx_set_option(int key, int64_t value)
         return cyl_set_path(value);
     case AGX_OPT_ADJ_STAGE:
         return adj_set_stage(value);
+    case AGX_OPT_MP_TRIAXIAL:
+        if (value < 0 || value > 1) return AGX_E_SIZE;
+        g_mp_triaxial = (int)value;
+        return AGX_OK;
     default:
         return AGX_E_WHAT;
     }

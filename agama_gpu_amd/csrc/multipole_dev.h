@@ -137,12 +137,27 @@ struct NoHook {
     __device__ void sums(double, double, double, double, double) const {}
 };
 
+// The triaxial program of even degree L (a table with every harmonic of even
+// l <= L and even m in 0..l, cos terms only, columns in program order: the
+// shape of every table with triaxial symmetry and no pruned term): group g
+// has m = 2g and the steps l = 2g, 2g + 2, ..., L; step j of the program
+// has its cos term in column j (step 0, l = m = 0, none: the monopole is
+// evaluated apart).  tri_step0(L, g) is the first step of group g.
+__host__ __device__ constexpr int tri_step0(int L, int g)
+{
+    return g * (L / 2 + 1) - g * (g - 1) / 2;
+}
+
 // Interior evaluation on radial interval k (slab = poly + k*n_lm*6).
 // UNI: the slab address is wave-uniform (scalar loads of coefficients).
 // STRIDE: 1 or 2 when every group of the program that recurs steps by that
 // stride (AGX_PROG_STRIDE1 / AGX_PROG_STRIDE2), 0 for any program; SIN false
 // when the program has no sin terms (AGX_PROG_COS_ONLY).
-template <int ORD, bool UNI, class Hook = NoHook, int STRIDE = 0, bool SIN = true>
+// TL > 0: the program is the triaxial program of degree TL (tri_program):
+// its groups, steps and column offsets are compile-time constants, so the
+// group and step loops unroll completely; the records supply only the
+// recurrence coefficients and q_mm.
+template <int ORD, bool UNI, class Hook = NoHook, int STRIDE = 0, bool SIN = true, int TL = 0>
 __device__ __forceinline__ void interior(const MPK &P, const double *slab, double x, double y,
                                          double z, double r2, double r, double ir, double lnr,
                                          int k, PointOut &o, double inv4piG,
@@ -207,11 +222,21 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
     // Program records are read one iteration ahead (the zero records after
     // the groups and after the steps keep the look-ahead in bounds), so the
     // scalar-load latency overlaps the current step's arithmetic.
-    const int S0 = P.n_groups + 1;
-    ivec8 gr = cld_rec(P.prog, 0);
-    for (int g = 0; g < P.n_groups; ++g) {
-        const ivec8 gn = cld_rec(P.prog, g + 1);
-        const int m = gr.s0;
+    constexpr bool CT = TL > 0;
+    constexpr int kUnrollGroups = CT ? TL / 2 + 1 : 1, kUnrollPairs = CT ? TL / 4 + 1 : 1;
+    const int n_groups = CT ? TL / 2 + 1 : P.n_groups;
+    const int S0 = n_groups + 1;
+    // (CT: the records' addresses are constants; re-derived per point, their
+    // loads stay here instead of being hoisted out of the caller's loop)
+    const void *prog = CT ? sopaque(P.prog) : P.prog;
+    static_assert(!CT || (!UNI && STRIDE == 2 && !SIN), "the triaxial program: LDS slabs, stride 2, cos terms");
+    typedef __attribute__((address_space(3))) const double *lds_dptr;
+    lds_dptr lslab = CT ? (lds_dptr)slab : nullptr;
+    ivec8 gr = cld_rec(prog, 0);
+#pragma unroll kUnrollGroups
+    for (int g = 0; g < n_groups; ++g) {
+        const ivec8 gn = cld_rec(prog, g + 1);
+        const int m = CT ? 2 * g : gr.s0;
         if (m - cur_m == 2) {
             // two steps at once (tables with even-m symmetry): rotate by 2 phi
             const double c2 = fma(cph, cph, -sph * sph), s2 = 2.0 * cph * sph;
@@ -228,7 +253,7 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
             cm = t;
             ++cur_m;
         }
-        const double q = rec_lo(gr), fm = rec_hi(gr);       // qmm and m
+        const double q = rec_lo(gr), fm = CT ? (double)m : rec_hi(gr);   // qmm and m
         // Two recurrence buffers A and B are used alternately: each step
         // writes the new P_l over the older of the two, so no register copies
         // are needed.  With scalar (UNI) coefficient loads the cos-term
@@ -255,15 +280,38 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
         };
         // the cos and sin terms of one step (degree rec.s0, current Q, dQ);
         // the cos coefficients were loaded into ca beforehand
-        auto terms = [&](const ivec8 &rec, double Q, double dQ, const double ca[6]) {
-            const int l = rec.s0;
-            const int oc = rec_off_lo(rec), os = rec_off_hi(rec);
+        // Steps j0 .. j1-1 (the first has l = m); coff: the cos offset of step
+        // j (record rec), -1 when it has no cos term
+        const int j0 = CT ? tri_step0(TL, g) : gr.s1, j1 = CT ? tri_step0(TL, g + 1) : gr.s2;
+        auto coff = [&](const ivec8 &rec, int j) {
+            return CT ? (j > 0 ? 6 * j : -1) : rec_off_lo(rec);
+        };
+        auto terms = [&](const ivec8 &rec, int j, double Q, double dQ, const double ca[6]) {
+            const int l = CT ? m + 2 * (j - j0) : rec.s0;
+            const int oc = coff(rec, j), os = CT ? -1 : rec_off_hi(rec);
             double d2Q = 0.0;
             if (ANG2 && !pole)
                 d2Q = -ct * ist_loop * dQ -
                       ((double)(l * (l + 1)) - (double)(m * m) * ist_loop * ist_loop) * Q;
             const double ll1 = (double)(l * (l + 1));
-            if (oc >= 0) {
+            if (CT) {
+                // the slab is in LDS; the next step's loads depend on this
+                // step's sum, so that the unrolled steps do not all load first
+                // (every step's coefficients live at once: spills)
+                if (oc >= 0) {
+                    double ac[6];
+                    ldvec2_t *p = (ldvec2_t *)(lslab + oc);
+                    const dvec2 q0 = p[0], q1 = p[1], q2 = p[2];
+                    ac[0] = q0.x; ac[1] = q0.y; ac[2] = q1.x; ac[3] = q1.y; ac[4] = q2.x; ac[5] = q2.y;
+                    add_term(ac, Q, dQ, d2Q, ll1, Vc, Rc, Tc, RRc, TTc, RTc, Lc);
+                    asm volatile("" : "+v"(lslab) : "v"(Vc));
+                    if (R1) asm volatile("" : "+v"(lslab) : "v"(Rc));
+                    if (D1) asm volatile("" : "+v"(lslab) : "v"(Tc));
+                    if (D2) asm volatile("" : "+v"(lslab) : "v"(RRc));
+                    if (ANG2) asm volatile("" : "+v"(lslab) : "v"(TTc), "v"(RTc));
+                    if (ORD == ORD_DENS) asm volatile("" : "+v"(lslab) : "v"(Lc));
+                }
+            } else if (oc >= 0) {
                 if (UNI) {
                     add_term(ca, Q, dQ, d2Q, ll1, Vc, Rc, Tc, RRc, TTc, RTc, Lc);
                 } else {
@@ -304,30 +352,30 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
             if (D1) dQo = fma(t, dQn, fma(al * dcc, Qn, ga * dQo));
             Qo = fma(t, Qn, ga * Qo);
         };
-        // Steps j0 .. j1-1 (the first has l = m).  Records past the group are
-        // read (the next group's, or the zero record) but not used.
-        const int j0 = gr.s1, j1 = gr.s2;
+        // Records past the group are read (the next group's, or the zero
+        // record) but not used.
         double cA[6], cB[6];
-        const ivec8 r0 = cld_rec(P.prog, S0 + j0);
-        if (UNI && rec_off_lo(r0) >= 0) load6(rec_off_lo(r0), cA);
-        ivec8 r1 = cld_rec(P.prog, S0 + j0 + 1);
-        if (UNI && j0 + 1 < j1 && rec_off_lo(r1) >= 0) load6(rec_off_lo(r1), cB);
-        terms(r0, QA, dQA, cA);
+        const ivec8 r0 = cld_rec(prog, S0 + j0);
+        if (UNI && coff(r0, j0) >= 0) load6(coff(r0, j0), cA);
+        ivec8 r1 = cld_rec(prog, S0 + j0 + 1);
+        if (UNI && j0 + 1 < j1 && coff(r1, j0 + 1) >= 0) load6(coff(r1, j0 + 1), cB);
+        terms(r0, j0, QA, dQA, cA);
         int j = j0 + 1;
+#pragma unroll kUnrollPairs
         for (; j + 1 < j1; j += 2) {
-            const ivec8 r2 = cld_rec(P.prog, S0 + j + 1);
-            if (UNI && rec_off_lo(r2) >= 0) load6(rec_off_lo(r2), cA);
+            const ivec8 r2 = cld_rec(prog, S0 + j + 1);
+            if (UNI && coff(r2, j + 1) >= 0) load6(coff(r2, j + 1), cA);
             recur(r1, QB, dQB, QA, dQA);
-            terms(r1, QB, dQB, cB);
-            const ivec8 r3 = cld_rec(P.prog, S0 + j + 2);
-            if (UNI && j + 2 < j1 && rec_off_lo(r3) >= 0) load6(rec_off_lo(r3), cB);
+            terms(r1, j, QB, dQB, cB);
+            const ivec8 r3 = cld_rec(prog, S0 + j + 2);
+            if (UNI && j + 2 < j1 && coff(r3, j + 2) >= 0) load6(coff(r3, j + 2), cB);
             recur(r2, QA, dQA, QB, dQB);
-            terms(r2, QA, dQA, cA);
+            terms(r2, j + 1, QA, dQA, cA);
             r1 = r3;
         }
         if (j < j1) {
             recur(r1, QB, dQB, QA, dQA);
-            terms(r1, QB, dQB, cB);
+            terms(r1, j, QB, dQB, cB);
         }
         // ---- apply cos/sin(m phi) to the group sums ------------------------
         // acc + cos C + sin S, and cos S - sin C; without SIN (no sin terms in
@@ -704,7 +752,8 @@ struct MpBinKey {
     }
 };
 
-constexpr uint32_t kProgFlags = AGX_PROG_STRIDE1 | AGX_PROG_COS_ONLY | AGX_PROG_STRIDE2;
+constexpr uint32_t kProgFlags =
+    AGX_PROG_STRIDE1 | AGX_PROG_COS_ONLY | AGX_PROG_STRIDE2 | AGX_PROG_TRIAXIAL_MASK;
 
 // The stride a kernel variant may assume from the program flags: 1, 2, or 0
 // (any).  A program with no recurring group satisfies both flags.

@@ -1,0 +1,63 @@
+// The unrolled LDS-staged Multipole kernels of the triaxial programs
+// (interior() with TL > 0): degrees 2..10, every evaluation order, 256- and
+// 512-thread workgroups (tables up to 80 KB; larger ones run the generic
+// kernels).  Degree 12 is left to the generic kernels: unrolled, its 28 steps
+// ran 3 % (potential) to 13 % (Hessian) slower than the loop at N = 10M.
+// A translation unit of its own: these instantiations compile in parallel
+// with multipole.hip.
+#include "multipole_lds.h"
+
+namespace agx {
+
+template <int ORD, int BLOCK>
+static bool tri_launch(int tl, const MPK &P, unsigned grid, int64_t tb, const double *xyz,
+                       int64_t n, double *phi, double *force, double *deriv, double inv4piG,
+                       hipStream_t s)
+{
+    switch (tl) {
+#define AGX_TRI_CASE(L)                                                                        \
+    case L:                                                                                    \
+        launch_lds<ORD, BLOCK, 2, false, L>(P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s); \
+        return true;
+        AGX_TRI_CASE(2)
+        AGX_TRI_CASE(4)
+        AGX_TRI_CASE(6)
+        AGX_TRI_CASE(8)
+        AGX_TRI_CASE(10)
+#undef AGX_TRI_CASE
+    default:
+        return false;
+    }
+}
+
+template <int ORD>
+static bool tri_launch(int block, int tl, const MPK &P, unsigned grid, int64_t tb,
+                       const double *xyz, int64_t n, double *phi, double *force, double *deriv,
+                       double inv4piG, hipStream_t s)
+{
+    if (block == 256)
+        return tri_launch<ORD, 256>(tl, P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s);
+    if (block == 512)
+        return tri_launch<ORD, 512>(tl, P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s);
+    return false;
+}
+
+bool launch_lds_triaxial(int ord, int block, int tl, const MPK &P, unsigned grid, int64_t tb,
+                         const double *xyz, int64_t n, double *phi, double *force, double *deriv,
+                         double inv4piG, hipStream_t s)
+{
+    switch (ord) {
+    case ORD_POT:
+        return tri_launch<ORD_POT>(block, tl, P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s);
+    case ORD_GRAD:
+        return tri_launch<ORD_GRAD>(block, tl, P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s);
+    case ORD_HESS:
+        return tri_launch<ORD_HESS>(block, tl, P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s);
+    case ORD_DENS:
+        return tri_launch<ORD_DENS>(block, tl, P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s);
+    default:
+        return false;
+    }
+}
+
+}  // namespace agx

@@ -250,16 +250,38 @@ def build_legendre_program(lm_l: np.ndarray, lm_m: np.ndarray) -> LegendreProgra
         np.asarray(stride, np.int32), np.asarray(gamma, np.float64))
 
 
+def triaxial_degree(prog: LegendreProgram) -> int:
+    """L when the program is the triaxial program of even degree L >= 2:
+    groups m = 0, 2, ..., L, group m stepping l = m, m + 2, ..., L by 2, cos
+    terms only, the cos term of step j in column j for j >= 1 (step 0, l = m =
+    0, has none: the monopole is evaluated apart) -- the program of a table
+    with every harmonic that triaxial symmetry allows up to degree L.
+    0 otherwise."""
+    G = len(prog.grp_m)
+    L = 2 * (G - 1)
+    if G < 2 or not np.array_equal(prog.grp_m, np.arange(0, L + 1, 2)):
+        return 0
+    step_l = np.concatenate([np.arange(m, L + 1, 2) for m in range(0, L + 1, 2)])
+    S = len(step_l)
+    step0 = np.concatenate([[0], np.cumsum([(L - m) // 2 + 1 for m in range(0, L + 1, 2)])])
+    ok = (np.array_equal(prog.grp_step0, step0) and np.array_equal(prog.step_l, step_l)
+          and np.array_equal(prog.step_cos, np.concatenate([[-1], np.arange(1, S)]))
+          and np.all(prog.step_sin < 0) and np.all(prog.grp_stride == 2))
+    return L if ok else 0
+
+
 def program_flags(prog: LegendreProgram) -> int:
     """agx_multipole.prog_flags for this program: AGX_PROG_STRIDE1 (1) /
     AGX_PROG_STRIDE2 (4) when every group that recurs (more than one step)
-    has stride 1 / 2, plus AGX_PROG_COS_ONLY (2) when no step has a sin term."""
+    has stride 1 / 2, plus AGX_PROG_COS_ONLY (2) when no step has a sin term,
+    plus the triaxial degree (triaxial_degree) in bits 8-15."""
     steps = np.diff(prog.grp_step0)
     recurs = steps > 1
     stride1 = not np.any((prog.grp_stride != 1) & recurs)
     stride2 = not np.any((prog.grp_stride != 2) & recurs)
     cos_only = not np.any(prog.step_sin >= 0)
-    return int(stride1) | (2 if cos_only else 0) | (4 if stride2 else 0)
+    return (int(stride1) | (2 if cos_only else 0) | (4 if stride2 else 0)
+            | (triaxial_degree(prog) << 8))
 
 
 # 32 bytes: four int32 (group records: m, step_begin, step_end, stride; step

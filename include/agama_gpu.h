@@ -80,12 +80,27 @@ enum {
  *             AGX_PROG_COS_ONLY no step of `program` has a sin term (every
  *                               sin_off is -1: tables with m >= 0 only);
  *             the kernels then run variants without the per-group stride
- *             arithmetic or the sin sums.  0 is always correct; a flag set
- *             for a program that does not satisfy it gives wrong results.
+ *             arithmetic or the sin sums;
+ *             plus L << AGX_PROG_TRIAXIAL_SHIFT (bits 8-15, even L >= 2)
+ *             when `program` is the triaxial program of degree L: groups
+ *             m = 0, 2, .. L, group m with the steps l = m, m + 2, .. L
+ *             (stride 2), cos terms only, cos_off = 6 j for step j >= 1 and
+ *             -1 for step 0 (l = m = 0) -- the program of a table with
+ *             every harmonic of even l <= L and even m in 0..l, columns in
+ *             program order (multipole_prep.triaxial_degree).  Requires
+ *             n_groups = L/2 + 1 and n_lm = n_steps = (L/2 + 1)(L/2 + 2)/2
+ *             (checked: AGX_E_WHAT otherwise); the LDS-staged path then
+ *             runs a kernel with that program unrolled at compile time
+ *             (degrees 2..10, tables up to 80 KB; AGX_OPT_MP_TRIAXIAL).
+ *             0 is always correct; a flag set for a program that does not
+ *             satisfy it gives wrong results.  No counterpart in the
+ *             reference (its kernel runs every table through the same loop).
  * ---------------------------------------------------------------------- */
 #define AGX_PROG_STRIDE1 1u
 #define AGX_PROG_COS_ONLY 2u
 #define AGX_PROG_STRIDE2 4u
+#define AGX_PROG_TRIAXIAL_SHIFT 8
+#define AGX_PROG_TRIAXIAL_MASK 0xFF00u
 
 typedef struct agx_multipole {
     const double *poly_km;
@@ -439,10 +454,15 @@ int agx_leapfrog(const agx_component *d_leaves, int32_t n_leaves,
                                  and summed in one pass where the staging
                                  blocks and the slabs fit in 53 KiB per
                                  workgroup, 0 six cross-lane sums per term */
+#define AGX_OPT_MP_TRIAXIAL 12 /* LDS-staged Multipole path: 1 (default) the
+                                 unrolled kernels of triaxial programs (the
+                                 degree in prog_flags bits 8-15: 2..10, tables
+                                 up to 80 KB) where the flags name one, 0 the
+                                 generic kernels for every program */
 int agx_set_option(int key, int64_t value);
 
 /* Library identification and error text.  agx_version: major * 10000 +
- * minor * 100 + patch (10400: this header).  1.2.0: agx_component.prog_flags
+ * minor * 100 + patch (10500: this header).  1.2.0: agx_component.prog_flags
  * (formerly padding) is read; a position with a NaN coordinate gives NaN
  * outputs, and one whose r^2 overflows (|x| above about 1.3e154) gives the
  * outer-law limit 0 for Multipole tables.  1.3.0: agx_multipole.origin (a new
@@ -450,7 +470,12 @@ int agx_set_option(int key, int64_t value);
  * rebuilt) and agx_multipole_adjoint's AGX_ADJ_CENTER_GRAD flag.  1.4.0: the
  * AGX_OPT_ADJ_STAGE option; CylSpline coordinates and values near the z axis
  * and the midplane, and DiskAnsatz profiles near the midplane, to full
- * relative precision (results change at the rounding level elsewhere). */
+ * relative precision (results change at the rounding level elsewhere).
+ * 1.5.0: prog_flags bits 8-15 (the triaxial degree; validated against the
+ * program's size) and AGX_OPT_MP_TRIAXIAL; the LDS-staged table's slabs are
+ * padded to an odd number of 16-byte words (fewer LDS bank conflicts for
+ * tables with an even number of terms; the LDS path admits tables up to
+ * 160 KB including the padding). */
 int agx_version(void);
 const char *agx_error_string(int code);
 

@@ -35,7 +35,7 @@ def test_library_exports_all_declared_symbols():
 def test_version_and_error_strings():
     from agama_gpu_amd import _lib
     L = _lib.load()
-    assert L.agx_version() == 10400
+    assert L.agx_version() == 10500
     assert b"NULL" in L.agx_error_string(-1)
     assert L.agx_error_string(12345) == b"unknown error code"
 
@@ -66,6 +66,17 @@ def test_argument_validation_without_gpu():
     assert L.agx_multipole_density(C.byref(d), None, 10, 1.0, None, None) == -5
     d.prog_flags = 0
     assert L.agx_multipole_density(C.byref(d), None, 10, 1.0, None, None) == -1
+    # a triaxial degree (bits 8-15) must be even and agree with the program's
+    # size: degree 8 is 5 groups, 15 steps and 15 columns
+    d.n_groups, d.n_steps, d.n_lm = 5, 15, 15
+    for tl, ok in ((8, True), (7, False), (6, False), (10, False)):
+        d.prog_flags = (tl << _lib.AGX_PROG_TRIAXIAL_SHIFT) | _lib.AGX_PROG_COS_ONLY | \
+            _lib.AGX_PROG_STRIDE2
+        assert L.agx_multipole_eval(C.byref(d), None, 10, None, None, None, 3, None) == \
+            (-1 if ok else -5), tl
+    d.n_lm = 14
+    assert L.agx_multipole_eval(C.byref(d), None, 10, None, None, None, 3, None) == -5
+    d.prog_flags = 0
 
 
 def test_component_flags_validation_without_gpu():
@@ -79,6 +90,9 @@ def test_component_flags_validation_without_gpu():
     comps[0].ampl = comps[0].scale = 1.0
     comps[0].prog_flags = _lib.AGX_PROG_COS_ONLY | _lib.AGX_PROG_STRIDE2
     assert L.agx_composite_eval(comps, 1, None, 0, None, None, None, 1, None) == 0
+    assert L.agx_composite_eval(comps, 1, None, 10, None, None, None, 1, None) == -1
+    # the triaxial degree is accepted (the composite kernel does not use it)
+    comps[0].prog_flags = (8 << 8) | _lib.AGX_PROG_COS_ONLY | _lib.AGX_PROG_STRIDE2
     assert L.agx_composite_eval(comps, 1, None, 10, None, None, None, 1, None) == -1
     comps[0].prog_flags = 8                # no such program flag
     assert L.agx_composite_eval(comps, 1, None, 10, None, None, None, 1, None) == -5

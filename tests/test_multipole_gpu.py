@@ -130,11 +130,14 @@ def test_program_flag_variants_agree(cases, name):
     from agama_gpu_amd.multipole_prep import program_flags
     gpu = MultipolePotentialGPU(cases[name])
     prog = gpu._data["program"]
-    want = (_lib.AGX_PROG_COS_ONLY | _lib.AGX_PROG_STRIDE2) if name == "halo_lmax8" \
-        else _lib.AGX_PROG_STRIDE1
+    # (halo_lmax8 is also the triaxial program of degree 8: bits 8-15, whose
+    # unrolled kernel tests/test_triaxial_gpu.py checks; off here)
+    want = (_lib.AGX_PROG_COS_ONLY | _lib.AGX_PROG_STRIDE2 | (8 << 8)) \
+        if name == "halo_lmax8" else _lib.AGX_PROG_STRIDE1
     assert program_flags(prog) == want == gpu._desc.prog_flags
     x = torch.as_tensor(points(100_000, 4), device="cuda")
     _lib.set_option(_lib.AGX_OPT_MP_PATH, 3)
+    _lib.set_option(_lib.AGX_OPT_MP_TRIAXIAL, 0)
     try:
         outs = []
         for flags in (gpu._desc.prog_flags, 0):
@@ -143,6 +146,7 @@ def test_program_flag_variants_agree(cases, name):
     finally:
         gpu._desc.prog_flags = want
         _lib.set_option(_lib.AGX_OPT_MP_PATH, 0)
+        _lib.set_option(_lib.AGX_OPT_MP_TRIAXIAL, 1)
     for a, b in zip(*outs):
         assert torch.equal(a, b)
 

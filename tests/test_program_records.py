@@ -93,3 +93,46 @@ def test_records_reproduce_legendre(lmax, sym):
                 h = 1e-5
                 dw = (want(ll, mm, th + h) - want(ll, mm, th - h)) / (2 * h)
                 assert abs(dQ1 - dw) <= 1e-7 * max(1.0, abs(dw)), (ll, mm, th, dQ1, dw)
+
+
+@pytest.mark.parametrize("lmax", [2, 4, 6, 8, 10, 12, 32])
+def test_triaxial_degree_of_triaxial_programs(lmax):
+    """Every even-even m >= 0 harmonic up to an even lmax: the triaxial
+    program of that degree, whose group g starts at step tri_step0(L, g) =
+    g (L/2 + 1) - g (g - 1) / 2 (the kernels' compile-time layout), with the
+    cos term of step j in column j; in program_flags bits 8-15."""
+    from agama_gpu_amd.multipole_prep import program_flags, triaxial_degree
+    l, m, prog = _program(lmax, "triaxial")
+    assert triaxial_degree(prog) == lmax
+    assert program_flags(prog) >> 8 == lmax and program_flags(prog) & 0xFF == 2 | 4
+    G = lmax // 2 + 1
+    assert list(prog.grp_step0) == [g * (lmax // 2 + 1) - g * (g - 1) // 2 for g in range(G + 1)]
+    assert list(prog.step_cos) == [-1] + list(range(1, len(prog.step_l)))
+    # an odd lmax adds nothing that triaxial symmetry allows
+    if lmax < 32:
+        assert triaxial_degree(_program(lmax + 1, "triaxial")[2]) == lmax
+
+
+def test_triaxial_degree_rejects_other_programs():
+    """A dense program, one with a term missing (the kernels would read the
+    wrong column), one with a sin term, one with an odd-l term, the monopole
+    alone: no triaxial degree."""
+    from agama_gpu_amd.expansion import lm_labels
+    from agama_gpu_amd.multipole_prep import (_sort_key_order, build_legendre_program,
+                                              triaxial_degree)
+
+    def prog_of(keep, lmax=8):
+        lab = [(a, b) for a, b in lm_labels(lmax) if keep(a, b)]
+        l = np.array([a for a, _ in lab], np.int32)
+        m = np.array([b for _, b in lab], np.int32)
+        o = _sort_key_order(l, m)
+        return build_legendre_program(l[o], m[o])
+
+    tri = lambda a, b: a % 2 == 0 and b % 2 == 0 and b >= 0          # noqa: E731
+    assert triaxial_degree(prog_of(tri)) == 8
+    assert triaxial_degree(prog_of(lambda a, b: True)) == 0
+    assert triaxial_degree(prog_of(lambda a, b: tri(a, b) and (a, b) != (6, 4))) == 0
+    assert triaxial_degree(prog_of(lambda a, b: tri(a, b) and (a, b) != (8, 8))) == 0
+    assert triaxial_degree(prog_of(lambda a, b: tri(a, b) or (a, b) == (4, -2))) == 0
+    assert triaxial_degree(prog_of(lambda a, b: tri(a, b) or (a, b) == (3, 0))) == 0
+    assert triaxial_degree(prog_of(lambda a, b: a == 0)) == 0
