@@ -209,7 +209,12 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
     const bool pole = st < 1.0e-10;
     // cos^2 theta and its theta derivative, once per point when every
     // recurring group steps by 2
-    const double ct2 = STRIDE == 2 ? ct * ct : 0.0, dct2 = STRIDE == 2 ? -2.0 * ct * st : 0.0;
+    // (unrolled, TL > 0: cos^2 kept opaque, or each step's cos^2 + beta
+    // contracts into an FMA of cos * cos whose accumulator, beta, must first
+    // be copied from scalar registers: two moves per step)
+    double ct2 = STRIDE == 2 ? ct * ct : 0.0;
+    if (TL > 0) ct2 = vopaque(ct2);
+    const double dct2 = STRIDE == 2 ? -2.0 * ct * st : 0.0;
     // 1/sin(theta) = r/R, needed inside the term loop only for the Hessian
     const double ist_loop = (ANG2 && !pole) ? r * iR : 0.0;
 
