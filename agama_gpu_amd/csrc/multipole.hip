@@ -1,5 +1,6 @@
 // Multipole kernels and C-ABI entry points (gfx950, FP64).
 #include <mutex>
+#include <type_traits>
 #include "multipole_dev.h"
 #include "multipole_lds.h"
 #include "sort_dev.h"
@@ -216,9 +217,10 @@ static void launch_mp(const agx_multipole *mp, const MPK &P, const double *xyz, 
     if (path == 3 && !lds_ok) path = 1;
     if (path == 2 && mp->n_int + 1 > kMaxBins) path = 1;
     if (path == 3) {
-        // Workgroups per CU that fit in LDS, and a workgroup size that gives
-        // them 16 wavefronts together (4 per SIMD): 256 threads for tables
-        // up to 40 KB, 512 up to 80 KB, 1024 up to 160 KB.
+        // Workgroups per CU that fit in LDS (at most 8) and a workgroup size
+        // that gives them the wavefronts the kernel's registers allow
+        // (lds_block): 4 per SIMD for pot+force and density, 8 for the
+        // potential alone, 3 for the Hessian.
         int64_t fit = kLdsPerCu / tb;
         if (fit > 8) fit = 8;
         // The grid is g_mp_lds_grid times the workgroups that fit at once:
@@ -228,15 +230,28 @@ static void launch_mp(const agx_multipole *mp, const MPK &P, const double *xyz, 
         // oversubscribed and the dispatcher hands freed slots to new
         // workgroups; each one re-stages the table from L2.
         const int64_t per_cu = fit * g_mp_lds_grid;
-        if (fit >= 4)
-            launch_lds<ORD, 256>(mp, P, grid_for(n, (int)per_cu), tb, xyz, n, phi, force, deriv,
-                                 inv4piG, s);
-        else if (fit >= 2)
-            launch_lds<ORD, 512>(mp, P, lds_grid(n, 512, per_cu), tb, xyz, n, phi, force, deriv,
-                                 inv4piG, s);
-        else
-            launch_lds<ORD, 1024>(mp, P, lds_grid(n, 1024, per_cu), tb, xyz, n, phi, force, deriv,
-                                  inv4piG, s);
+        auto go = [&](auto block) {
+            constexpr int B = decltype(block)::value;
+            launch_lds<ORD, B>(mp, P, lds_grid(n, B, per_cu), tb, xyz, n, phi, force, deriv,
+                               inv4piG, s);
+        };
+        using std::integral_constant;
+        if constexpr (ORD == ORD_POT) {
+            // at most 64 VGPRs: 8 waves per SIMD, 32 wavefronts per CU
+            if (fit >= 4) go(integral_constant<int, 512>{});
+            else go(integral_constant<int, 1024>{});
+        } else if constexpr (ORD == ORD_HESS) {
+            // 143-168 VGPRs: 3 waves per SIMD, 12 wavefronts per CU (a
+            // 1024-thread workgroup would cap it at 128 VGPRs: spills)
+            if (fit >= 3) go(integral_constant<int, 256>{});
+            else if (fit == 2) go(integral_constant<int, 384>{});
+            else go(integral_constant<int, 768>{});
+        } else {
+            // 4 waves per SIMD, 16 wavefronts per CU
+            if (fit >= 4) go(integral_constant<int, 256>{});
+            else if (fit >= 2) go(integral_constant<int, 512>{});
+            else go(integral_constant<int, 1024>{});
+        }
     } else if (path == 2) {
         BinLaunch bl = bin_launch(n);
         hipLaunchKernelGGL(mp_binned_kernel<ORD>, dim3(bl.grid), dim3(kBinThreads), 0, s, P, xyz,

@@ -22,9 +22,15 @@ __host__ __device__ inline int lds_slab_words(int n_lm) { return 3 * n_lm + (n_l
 // TL > 0: the triaxial program of degree TL, unrolled (interior()); its
 // register use is held to the occupancy the LDS allows anyway (4 waves per
 // SIMD, 3 for the Hessian), or the unrolled steps' loads would all be
-// scheduled first.
+// scheduled first.  The potential alone is held to 64 VGPRs (8 waves: it
+// runs in workgroups twice as large).
+template <int ORD, int BLOCK, int STRIDE, bool SIN, int TL>
+constexpr int lds_min_waves()
+{
+    return ORD == ORD_POT ? 8 : TL == 0 ? 1 : ORD == ORD_HESS ? 3 : 4;
+}
 template <int ORD, int BLOCK, int STRIDE, bool SIN, int TL = 0>
-__global__ __launch_bounds__(BLOCK, TL > 0 ? (ORD == ORD_HESS ? 3 : 4) : 1) void mp_lds_kernel(
+__global__ __launch_bounds__(BLOCK, (lds_min_waves<ORD, BLOCK, STRIDE, SIN, TL>())) void mp_lds_kernel(
     MPK P, const double *__restrict__ xyz, int64_t n, double *__restrict__ phi,
     double *__restrict__ force, double *__restrict__ deriv, double inv4piG)
 {
@@ -99,7 +105,7 @@ static void launch_lds(const MPK &P, unsigned grid, int64_t tb, const double *xy
 }
 
 // The unrolled kernel of the triaxial program of degree tl (even, 2..10) for
-// this evaluation order and workgroup size (256 or 512 threads); false
+// this evaluation order and workgroup size (tables up to 80 KB); false
 // (nothing launched) where no such variant is built: the caller then runs
 // the generic one.  multipole_tri.hip.
 bool launch_lds_triaxial(int ord, int block, int tl, const MPK &P, unsigned grid, int64_t tb,

@@ -1,6 +1,6 @@
 // The unrolled LDS-staged Multipole kernels of the triaxial programs
-// (interior() with TL > 0): degrees 2..10, every evaluation order, 256- and
-// 512-thread workgroups (tables up to 80 KB; larger ones run the generic
+// (interior() with TL > 0): degrees 2..10, every evaluation order, in the
+// workgroup sizes of tables up to 80 KB (larger ones run the generic
 // kernels).  Degree 12 is left to the generic kernels: unrolled, its 28 steps
 // ran 3 % (potential) to 13 % (Hessian) slower than the loop at N = 10M.
 // A translation unit of its own: these instantiations compile in parallel
@@ -30,15 +30,20 @@ static bool tri_launch(int tl, const MPK &P, unsigned grid, int64_t tb, const do
     }
 }
 
+// The workgroup sizes the LDS path uses for tables up to 80 KB (the
+// generic dispatch in multipole.hip): 512 and 1024 threads for the
+// potential alone, 256 and 384 for the Hessian, 256 and 512 otherwise.
 template <int ORD>
 static bool tri_launch(int block, int tl, const MPK &P, unsigned grid, int64_t tb,
                        const double *xyz, int64_t n, double *phi, double *force, double *deriv,
                        double inv4piG, hipStream_t s)
 {
-    if (block == 256)
-        return tri_launch<ORD, 256>(tl, P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s);
-    if (block == 512)
-        return tri_launch<ORD, 512>(tl, P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s);
+    constexpr int B1 = ORD == ORD_POT ? 512 : 256;
+    constexpr int B2 = ORD == ORD_POT ? 1024 : ORD == ORD_HESS ? 384 : 512;
+    if (block == B1)
+        return tri_launch<ORD, B1>(tl, P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s);
+    if (block == B2)
+        return tri_launch<ORD, B2>(tl, P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s);
     return false;
 }
 
