@@ -75,9 +75,13 @@ class MultipoleEnsembleGPU:
         self._prog_flags = 0xFFFFFFFF
         for p in self.potentials:
             self._prog_flags &= int(p.descriptor.prog_flags)
-        # (the ensemble kernel has no unrolled triaxial variants)
+        # (the triaxial degrees, which differ between potentials, go with
+        # each potential's items instead: bits 8-15 of an LDS item's path)
         self._prog_flags &= ~_lib.AGX_PROG_TRIAXIAL_MASK & 0xFFFFFFFF
         path = np.array([c[0] for c in self.choices], np.int32)
+        for j, p in enumerate(self.potentials):
+            if path[j] == _lib.AGX_BATCH_LDS:
+                path[j] |= int(p.descriptor.prog_flags) & _lib.AGX_PROG_TRIAXIAL_MASK
         chunk = np.array([c[1] for c in self.choices], np.int32)
         cost = np.array([_cost_per_particle(p) for p in self.potentials], np.float64)
         n_items = C.c_int64(0)

@@ -29,8 +29,9 @@ constexpr int kBatchTableDoubles = (160 * 1024 - 256) / 8;
 __global__ void batch_queue_reset(int *q) { if (threadIdx.x == 0) q[0] = 0; }
 
 // rows [start, start + count) with the table at tab_base; each thread reads
-// its next position one iteration ahead
-template <int ORD, int STRIDE, bool SIN>
+// its next position one iteration ahead.  TL > 0: the potential's program is
+// the triaxial program of degree TL and its table is in LDS (interior()).
+template <int ORD, int STRIDE, bool SIN, int TL = 0>
 __device__ __forceinline__ void batch_rows(const MPK &P, const double *tab_base,
                                            const double *__restrict__ xyz, int64_t start, int count,
                                            double *phi, double *force, double *deriv)
@@ -51,8 +52,9 @@ __device__ __forceinline__ void batch_rows(const MPK &P, const double *tab_base,
         PointOut o{};
         Where w = classify(P, x, y, z);
         if (w.kind == 2)
-            interior<ORD, false, NoHook, STRIDE, SIN>(P, tab_base + (int64_t)w.k * P.n_lm * 6, x,
-                                                       y, z, w.r2, w.r, w.ir, w.lnr, w.k, o, 0.0);
+            interior<ORD, false, NoHook, TL ? 2 : STRIDE, TL ? false : SIN, TL>(
+                P, tab_base + (int64_t)w.k * P.n_lm * 6, x, y, z, w.r2, w.r, w.ir, w.lnr, w.k, o,
+                0.0);
         else
             extrap<ORD>(P, w, x, y, z, o, 0.0);
         store<ORD>(o, start + j, phi, force, deriv);
@@ -84,12 +86,31 @@ __global__ __launch_bounds__(kBatchBlock) void mp_batch_kernel(
         if (start < 0 || count < 1 || start > n_rows - count || pot < 0 || pot >= n_pot) continue;
         const MPK P = with_law(load_desc<MPK>(descs + (int64_t)pot * stride));
         const int tab_len = P.n_int * P.n_lm * 6;
-        if (path == AGX_BATCH_LDS && tab_len <= kBatchTableDoubles) {
+        if ((path & 0xFF) == AGX_BATCH_LDS && tab_len <= kBatchTableDoubles) {
             const dvec2 *src = (const dvec2 *)P.poly;
             dvec2 *dst = (dvec2 *)tab;
             for (int j = tid; j < tab_len / 2; j += kBatchBlock) dst[j] = src[j];
             __syncthreads();
-            batch_rows<ORD, STRIDE, SIN>(P, tab, xyz, start, count, phi, force, deriv);
+            // the triaxial degree in bits 8-15 of the item's path, where it
+            // agrees with the descriptor's sizes: the unrolled program (not
+            // for the Hessian, which the 1024-thread workgroups hold to 128
+            // VGPRs already)
+            const int tl = (path >> 8) & 0xFF;
+            const bool tri = ORD != ORD_HESS && tl >= 2 && tl <= 10 && !(tl & 1) &&
+                             P.n_groups == tl / 2 + 1 && P.n_steps == tri_step0(tl, tl / 2 + 1) &&
+                             P.n_lm == P.n_steps;
+            if constexpr (ORD != ORD_HESS) {
+                if (tri) {
+                    switch (tl) {
+                    case 2: batch_rows<ORD, STRIDE, SIN, 2>(P, tab, xyz, start, count, phi, force, deriv); break;
+                    case 4: batch_rows<ORD, STRIDE, SIN, 4>(P, tab, xyz, start, count, phi, force, deriv); break;
+                    case 6: batch_rows<ORD, STRIDE, SIN, 6>(P, tab, xyz, start, count, phi, force, deriv); break;
+                    case 8: batch_rows<ORD, STRIDE, SIN, 8>(P, tab, xyz, start, count, phi, force, deriv); break;
+                    default: batch_rows<ORD, STRIDE, SIN, 10>(P, tab, xyz, start, count, phi, force, deriv); break;
+                    }
+                }
+            }
+            if (!tri) batch_rows<ORD, STRIDE, SIN>(P, tab, xyz, start, count, phi, force, deriv);
             __syncthreads();
         } else {
             batch_rows<ORD, STRIDE, SIN>(P, P.poly, xyz, start, count, phi, force, deriv);
@@ -163,7 +184,11 @@ extern "C" int agx_batch_plan(int32_t n_pot, const int64_t *counts, const int32_
     int64_t total = 0;
     for (int32_t p = 0; p < n_pot; ++p) {
         if (counts[p] < 0 || chunk[p] < 1 || chunk[p] > AGX_BATCH_MAX_ITEM) return AGX_E_SIZE;
-        if (path[p] < AGX_BATCH_LANE || path[p] > AGX_BATCH_LDS) return AGX_E_WHAT;
+        // AGX_BATCH_LANE or AGX_BATCH_LDS, the latter with a triaxial degree
+        // (even, <= 32) in bits 8-15
+        const int base = path[p] & 0xFF, tl = path[p] >> 8;
+        if (base < AGX_BATCH_LANE || base > AGX_BATCH_LDS) return AGX_E_WHAT;
+        if (tl < 0 || tl > 32 || (tl & 1) || (tl && base != AGX_BATCH_LDS)) return AGX_E_WHAT;
         total += (counts[p] + chunk[p] - 1) / chunk[p];
     }
     *n_items = total;

@@ -33,7 +33,7 @@ namespace agx {
 struct MPK {
     const double *poly;              // [n_int][n_lm][6]
     const void *prog;                // Legendre program records (agx_multipole.program)
-    int n_int, n_lm, n_groups, log_scaling, inner_log, outer_log;
+    int n_int, n_lm, n_groups, n_steps, log_scaling, inner_log, outer_log;
     double logr_min, dlogr, inv_dlogr, invPhi0;
     double inner_s, inner_U, inner_W, outer_s, outer_U, outer_W;
     double inv_rmin, inv_rmax;       // 1 / r at the first and last node
@@ -772,7 +772,7 @@ inline MPK make_args(const agx_multipole *mp)
     MPK P;
     P.poly = mp->poly_km;
     P.prog = mp->program;
-    P.n_int = mp->n_int; P.n_lm = mp->n_lm; P.n_groups = mp->n_groups;
+    P.n_int = mp->n_int; P.n_lm = mp->n_lm; P.n_groups = mp->n_groups; P.n_steps = mp->n_steps;
     P.log_scaling = mp->log_scaling; P.inner_log = mp->inner_log; P.outer_log = mp->outer_log;
     P.logr_min = mp->logr_min; P.dlogr = mp->dlogr; P.inv_dlogr = mp->inv_dlogr;
     P.invPhi0 = mp->invPhi0;

@@ -346,8 +346,10 @@ int64_t agx_cylspline_adjoint_work_size(const agx_cylspline *cs, int64_t n);
  *
  * agx_batch_plan (host only) cuts potential p's counts[p] rows (ranges are
  * consecutive, in potential order) into items of at most chunk[p] rows, with
- * path[p], and orders them by cost[p] * rows, largest first.  With
- * items == NULL it only returns the item count in *n_items.
+ * path[p] (AGX_BATCH_LANE, or AGX_BATCH_LDS plus an even triaxial degree up
+ * to 32 in bits 8-15; AGX_E_WHAT otherwise), and orders them by
+ * cost[p] * rows, largest first.  With items == NULL it only returns the
+ * item count in *n_items.
  *
  * agx_multipole_batch_eval evaluates the items: d_desc holds n_pot descriptor
  * images (agx_multipole_pack) at desc_stride bytes apart; d_items the plan;
@@ -369,7 +371,11 @@ typedef struct {
     int64_t start;           /* first row */
     int32_t count;           /* rows, 1..AGX_BATCH_MAX_ITEM */
     int32_t pot;             /* potential index */
-    int32_t path;            /* AGX_BATCH_* */
+    int32_t path;            /* AGX_BATCH_*; for AGX_BATCH_LDS, bits 8-15 may
+                                hold the potential's triaxial degree (its
+                                prog_flags bits 8-15): degrees 2..10 run the
+                                unrolled program for the potential and for
+                                pot+force where the descriptor's sizes agree */
     int32_t pad;
 } agx_batch_item;
 

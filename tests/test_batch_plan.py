@@ -53,8 +53,11 @@ def test_plan_covers_every_row_once():
 
 
 @pytest.mark.parametrize("bad,code", [("chunk0", -2), ("chunk_big", -2), ("path", -5),
-                                      ("count", -2)])
+                                      ("count", -2), ("odd_degree", -5), ("lane_degree", -5),
+                                      ("big_degree", -5), ("high_bits", -5), ("degree", 0)])
 def test_plan_rejects(bad, code):
+    """Item paths: AGX_BATCH_LANE, or AGX_BATCH_LDS with an even triaxial
+    degree up to 32 in bits 8-15 (carried to the items unchanged)."""
     from agama_gpu_amd import _lib
     counts, chunk, path, cost = [10, 20], [8, 8], [0, 1], [1.0, 1.0]
     if bad == "chunk0":
@@ -63,6 +66,16 @@ def test_plan_rejects(bad, code):
         chunk = [8, _lib.AGX_BATCH_MAX_ITEM + 1]
     elif bad == "path":
         path = [0, 2]
+    elif bad == "odd_degree":
+        path = [0, 1 | (7 << 8)]
+    elif bad == "lane_degree":
+        path = [8 << 8, 1]
+    elif bad == "big_degree":
+        path = [0, 1 | (34 << 8)]
+    elif bad == "high_bits":
+        path = [0, 1 | (1 << 16)]
+    elif bad == "degree":
+        path = [0, 1 | (8 << 8)]
     else:
         counts = [10, -1]
     rc, _ = _plan(counts, chunk, path, cost)
