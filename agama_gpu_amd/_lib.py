@@ -79,6 +79,7 @@ AGX_PROG_STRIDE1 = 1
 AGX_PROG_COS_ONLY = 2
 AGX_PROG_STRIDE2 = 4
 AGX_PROG_TRIAXIAL_SHIFT, AGX_PROG_TRIAXIAL_MASK = 8, 0xFF00
+AGX_PROG_FULL_SHIFT, AGX_PROG_FULL_MASK = 16, 0xFF0000
 AGX_OPT_MP_PATH, AGX_OPT_MP_TILE, AGX_OPT_MP_BIN_MIN_N, AGX_OPT_MP_BIN_MIN_TERMS = 1, 2, 3, 4
 AGX_OPT_MP_LDS_GRID = 6
 AGX_OPT_MP_BIN_GRID = 7

@@ -77,7 +77,7 @@ class MultipoleEnsembleGPU:
             self._prog_flags &= int(p.descriptor.prog_flags)
         # (the triaxial degrees, which differ between potentials, go with
         # each potential's items instead: bits 8-15 of an LDS item's path)
-        self._prog_flags &= ~_lib.AGX_PROG_TRIAXIAL_MASK & 0xFFFFFFFF
+        self._prog_flags &= ~(_lib.AGX_PROG_TRIAXIAL_MASK | _lib.AGX_PROG_FULL_MASK) & 0xFFFFFFFF
         path = np.array([c[0] for c in self.choices], np.int32)
         for j, p in enumerate(self.potentials):
             if path[j] == _lib.AGX_BATCH_LDS:

@@ -210,8 +210,8 @@ static int check_components(const agx_component *comps, int n_comps, CompositeAr
     A.defer = -1;
     any_expansion = false;
     // AND over the Multipole leaves (of the stride and sin flags: the
-    // composite kernel does not use the triaxial degree)
-    mp_flags = kProgFlags & ~AGX_PROG_TRIAXIAL_MASK;
+    // composite kernel does not use the triaxial or full degree)
+    mp_flags = kProgFlags & ~(AGX_PROG_TRIAXIAL_MASK | AGX_PROG_FULL_MASK);
     for (int c = 0; c < n_comps; ++c) {
         const agx_component &C = comps[c];
         const bool expansion = C.type == AGX_T_MULTIPOLE || C.type == AGX_T_CYLSPLINE;

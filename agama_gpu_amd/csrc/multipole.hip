@@ -123,6 +123,11 @@ static int check_mp(const agx_multipole *mp, int64_t n)
     if (tl && ((tl & 1) || mp->n_groups != tl / 2 + 1 || mp->n_steps != tri_step0(tl, tl / 2 + 1) ||
                mp->n_lm != mp->n_steps))
         return AGX_E_WHAT;
+    // likewise a full degree (and not both)
+    const int fl = (mp->prog_flags & AGX_PROG_FULL_MASK) >> AGX_PROG_FULL_SHIFT;
+    if (fl && (tl || mp->n_groups != fl + 1 || mp->n_steps != full_step0(fl, fl + 1) ||
+               mp->n_lm != (fl + 1) * (fl + 1)))
+        return AGX_E_WHAT;
     return AGX_OK;
 }
 
@@ -189,6 +194,10 @@ static void launch_lds(const agx_multipole *mp, const MPK &P, unsigned grid, int
     const int tl = (mp->prog_flags & AGX_PROG_TRIAXIAL_MASK) >> AGX_PROG_TRIAXIAL_SHIFT;
     if (tl && g_mp_triaxial &&
         launch_lds_triaxial(ORD, BLOCK, tl, P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s))
+        return;
+    const int fl = (mp->prog_flags & AGX_PROG_FULL_MASK) >> AGX_PROG_FULL_SHIFT;
+    if (fl && g_mp_triaxial &&
+        launch_lds_full(ORD, BLOCK, fl, P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s))
         return;
     if (st == 1 && co)
         launch_lds<ORD, BLOCK, 1, false>(P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s);

@@ -148,16 +148,28 @@ __host__ __device__ constexpr int tri_step0(int L, int g)
     return g * (L / 2 + 1) - g * (g - 1) / 2;
 }
 
+// The full program of degree L (a table with every harmonic up to l = L, cos
+// and sin, columns in program order: the shape of every table without
+// symmetry and no pruned term): group g has m = g and the steps l = g, g + 1,
+// .., L; a step j <= L (group 0) has its cos term in column j (step 0, l = 0,
+// none), a later step j its cos term in column 2j - L - 1 and its sin term in
+// column 2j - L.  full_step0(L, g) is the first step of group g.
+__host__ __device__ constexpr int full_step0(int L, int g)
+{
+    return g * (L + 1) - g * (g - 1) / 2;
+}
+
 // Interior evaluation on radial interval k (slab = poly + k*n_lm*6).
 // UNI: the slab address is wave-uniform (scalar loads of coefficients).
 // STRIDE: 1 or 2 when every group of the program that recurs steps by that
 // stride (AGX_PROG_STRIDE1 / AGX_PROG_STRIDE2), 0 for any program; SIN false
 // when the program has no sin terms (AGX_PROG_COS_ONLY).
-// TL > 0: the program is the triaxial program of degree TL (tri_program):
-// its groups, steps and column offsets are compile-time constants, so the
-// group and step loops unroll completely; the records supply only the
-// recurrence coefficients and q_mm.
-template <int ORD, bool UNI, class Hook = NoHook, int STRIDE = 0, bool SIN = true, int TL = 0>
+// TL > 0: the program is the triaxial program of degree TL (tri_step0), FL > 0
+// the full program of degree FL (full_step0): its groups, steps and column
+// offsets are compile-time constants, so the group and step loops unroll
+// completely; the records supply only the recurrence coefficients and q_mm.
+template <int ORD, bool UNI, class Hook = NoHook, int STRIDE = 0, bool SIN = true, int TL = 0,
+          int FL = 0>
 __device__ __forceinline__ void interior(const MPK &P, const double *slab, double x, double y,
                                          double z, double r2, double r, double ir, double lnr,
                                          int k, PointOut &o, double inv4piG,
@@ -227,21 +239,26 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
     // Program records are read one iteration ahead (the zero records after
     // the groups and after the steps keep the look-ahead in bounds), so the
     // scalar-load latency overlaps the current step's arithmetic.
-    constexpr bool CT = TL > 0;
-    constexpr int kUnrollGroups = CT ? TL / 2 + 1 : 1, kUnrollPairs = CT ? TL / 4 + 1 : 1;
-    const int n_groups = CT ? TL / 2 + 1 : P.n_groups;
+    constexpr bool CT = TL > 0 || FL > 0;
+    constexpr int kUnrollGroups = TL > 0 ? TL / 2 + 1 : FL > 0 ? FL + 1 : 1;
+    constexpr int kUnrollPairs = TL > 0 ? TL / 4 + 1 : FL > 0 ? FL / 2 + 1 : 1;
+    const int n_groups = CT ? kUnrollGroups : P.n_groups;
     const int S0 = n_groups + 1;
     // (CT: the records' addresses are constants; re-derived per point, their
     // loads stay here instead of being hoisted out of the caller's loop)
+    // (each unrolled step re-derives it again, below: the next steps' records
+    // are then read one step ahead, not all at the start)
     const void *prog = CT ? sopaque(P.prog) : P.prog;
-    static_assert(!CT || (!UNI && STRIDE == 2 && !SIN), "the triaxial program: LDS slabs, stride 2, cos terms");
+    static_assert(!(TL > 0 && FL > 0) && (!CT || !UNI), "one compile-time program, LDS slabs");
+    static_assert(TL == 0 || (STRIDE == 2 && !SIN), "the triaxial program: stride 2, cos terms");
+    static_assert(FL == 0 || (STRIDE == 1 && SIN), "the full program: stride 1, cos and sin terms");
     typedef __attribute__((address_space(3))) const double *lds_dptr;
     lds_dptr lslab = CT ? (lds_dptr)slab : nullptr;
     ivec8 gr = cld_rec(prog, 0);
 #pragma unroll kUnrollGroups
     for (int g = 0; g < n_groups; ++g) {
         const ivec8 gn = cld_rec(prog, g + 1);
-        const int m = CT ? 2 * g : gr.s0;
+        const int m = TL > 0 ? 2 * g : FL > 0 ? g : gr.s0;
         if (m - cur_m == 2) {
             // two steps at once (tables with even-m symmetry): rotate by 2 phi
             const double c2 = fma(cph, cph, -sph * sph), s2 = 2.0 * cph * sph;
@@ -267,6 +284,7 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
         // use, where prefetching costs more VGPRs than it saves in latency.
         double QA = q * sp, dQA = 0.0, QB = 0.0, dQB = 0.0;
         if (D1) dQA = fm * q * sp1 * ct;                     // 0 for m = 0
+
         double Vc = 0.0, Vs = 0.0, Rc = 0.0, Rs = 0.0, Tc = 0.0, Ts = 0.0;
         double RRc = 0.0, RRs = 0.0, TTc = 0.0, TTs = 0.0, RTc = 0.0, RTs = 0.0, Lc = 0.0, Ls = 0.0;
         auto add_term = [&](const double a[6], double Q, double dQ, double d2Q, double ll1, double &V,
@@ -287,13 +305,21 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
         // the cos coefficients were loaded into ca beforehand
         // Steps j0 .. j1-1 (the first has l = m); coff: the cos offset of step
         // j (record rec), -1 when it has no cos term
-        const int j0 = CT ? tri_step0(TL, g) : gr.s1, j1 = CT ? tri_step0(TL, g + 1) : gr.s2;
+        const int j0 = TL > 0 ? tri_step0(TL, g) : FL > 0 ? full_step0(FL, g) : gr.s1;
+        const int j1 = TL > 0 ? tri_step0(TL, g + 1) : FL > 0 ? full_step0(FL, g + 1) : gr.s2;
         auto coff = [&](const ivec8 &rec, int j) {
-            return CT ? (j > 0 ? 6 * j : -1) : rec_off_lo(rec);
+            if (TL > 0) return j > 0 ? 6 * j : -1;
+            if (FL > 0) return j == 0 ? -1 : j <= FL ? 6 * j : 6 * (2 * j - FL - 1);
+            return rec_off_lo(rec);
+        };
+        auto soff = [&](const ivec8 &rec, int j) {
+            if (TL > 0) return -1;
+            if (FL > 0) return j <= FL ? -1 : 6 * (2 * j - FL);
+            return rec_off_hi(rec);
         };
         auto terms = [&](const ivec8 &rec, int j, double Q, double dQ, const double ca[6]) {
-            const int l = CT ? m + 2 * (j - j0) : rec.s0;
-            const int oc = coff(rec, j), os = CT ? -1 : rec_off_hi(rec);
+            const int l = CT ? m + (TL > 0 ? 2 : 1) * (j - j0) : rec.s0;
+            const int oc = coff(rec, j), os = soff(rec, j);
             double d2Q = 0.0;
             if (ANG2 && !pole)
                 d2Q = -ct * ist_loop * dQ -
@@ -301,21 +327,30 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
             const double ll1 = (double)(l * (l + 1));
             if (CT) {
                 // the slab is in LDS; the next step's loads depend on this
-                // step's sum, so that the unrolled steps do not all load first
-                // (every step's coefficients live at once: spills)
-                if (oc >= 0) {
-                    double ac[6];
-                    ldvec2_t *p = (ldvec2_t *)(lslab + oc);
+                // step's sums, so that the unrolled steps do not all load
+                // first (every step's coefficients live at once: spills)
+                auto lds6 = [&](int off, double a[6]) {
+                    ldvec2_t *p = (ldvec2_t *)(lslab + off);
                     const dvec2 q0 = p[0], q1 = p[1], q2 = p[2];
-                    ac[0] = q0.x; ac[1] = q0.y; ac[2] = q1.x; ac[3] = q1.y; ac[4] = q2.x; ac[5] = q2.y;
-                    add_term(ac, Q, dQ, d2Q, ll1, Vc, Rc, Tc, RRc, TTc, RTc, Lc);
-                    asm volatile("" : "+v"(lslab) : "v"(Vc));
-                    if (R1) asm volatile("" : "+v"(lslab) : "v"(Rc));
-                    if (D1) asm volatile("" : "+v"(lslab) : "v"(Tc));
-                    if (D2) asm volatile("" : "+v"(lslab) : "v"(RRc));
-                    if (ANG2) asm volatile("" : "+v"(lslab) : "v"(TTc), "v"(RTc));
-                    if (ORD == ORD_DENS) asm volatile("" : "+v"(lslab) : "v"(Lc));
-                }
+                    a[0] = q0.x; a[1] = q0.y; a[2] = q1.x; a[3] = q1.y; a[4] = q2.x; a[5] = q2.y;
+                };
+                auto tie = [&](double V, double Rr, double Tt, double RRr, double TTt, double RTt,
+                               double LL) {
+                    asm volatile("" : "+v"(lslab), "+s"(prog) : "v"(V));
+                    if (R1) asm volatile("" : "+v"(lslab) : "v"(Rr));
+                    if (D1) asm volatile("" : "+v"(lslab) : "v"(Tt));
+                    if (D2) asm volatile("" : "+v"(lslab) : "v"(RRr));
+                    if (ANG2) asm volatile("" : "+v"(lslab) : "v"(TTt), "v"(RTt));
+                    if (ORD == ORD_DENS) asm volatile("" : "+v"(lslab) : "v"(LL));
+                };
+                double ac[6], as[6];
+                if (oc >= 0) lds6(oc, ac);
+                if (SIN && os >= 0) lds6(os, as);
+                if (oc >= 0) add_term(ac, Q, dQ, d2Q, ll1, Vc, Rc, Tc, RRc, TTc, RTc, Lc);
+                if (SIN && os >= 0) add_term(as, Q, dQ, d2Q, ll1, Vs, Rs, Ts, RRs, TTs, RTs, Ls);
+                if (oc >= 0) tie(Vc, Rc, Tc, RRc, TTc, RTc, Lc);
+                if (SIN && os >= 0) tie(Vs, Rs, Ts, RRs, TTs, RTs, Ls);
+
             } else if (oc >= 0) {
                 if (UNI) {
                     add_term(ca, Q, dQ, d2Q, ll1, Vc, Rc, Tc, RRc, TTc, RTc, Lc);
@@ -325,7 +360,7 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
                     add_term(ac, Q, dQ, d2Q, ll1, Vc, Rc, Tc, RRc, TTc, RTc, Lc);
                 }
             }
-            if (SIN && os >= 0) {
+            if (!CT && SIN && os >= 0) {
                 double as[6];
                 load6(os, as);
                 add_term(as, Q, dQ, d2Q, ll1, Vs, Rs, Ts, RRs, TTs, RTs, Ls);
@@ -413,6 +448,18 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
         }
         if (ORD == ORD_DENS) SL = comb(Lc, Ls, SL);
         gr = gn;
+        // unrolled: the running sums formed here, at the group's end (left to
+        // itself the compiler sinks every group's combination to the end of
+        // the program, holding each group's sums and cos/sin(m phi) until
+        // then: spills)
+        if (CT) {
+            asm volatile("" : "+v"(Sv));
+            if (R1) asm volatile("" : "+v"(Sr));
+            if (D1) asm volatile("" : "+v"(St), "+v"(Sfn), "+v"(Sfp));
+            if (D2) asm volatile("" : "+v"(Srr));
+            if (ANG2) asm volatile("" : "+v"(Stt), "+v"(Srt), "+v"(Sf), "+v"(Sff), "+v"(Srf), "+v"(Stf));
+            if (ORD == ORD_DENS) asm volatile("" : "+v"(SL));
+        }
     }
 
     hook.sums(Sv, Sr, St, Sfn, Sfp);
@@ -757,8 +804,8 @@ struct MpBinKey {
     }
 };
 
-constexpr uint32_t kProgFlags =
-    AGX_PROG_STRIDE1 | AGX_PROG_COS_ONLY | AGX_PROG_STRIDE2 | AGX_PROG_TRIAXIAL_MASK;
+constexpr uint32_t kProgFlags = AGX_PROG_STRIDE1 | AGX_PROG_COS_ONLY | AGX_PROG_STRIDE2 |
+                                AGX_PROG_TRIAXIAL_MASK | AGX_PROG_FULL_MASK;
 
 // The stride a kernel variant may assume from the program flags: 1, 2, or 0
 // (any).  A program with no recurring group satisfies both flags.

@@ -19,18 +19,18 @@ __host__ __device__ inline int lds_slab_words(int n_lm) { return 3 * n_lm + (n_l
 // Used for tables of up to 160 KB (the LDS of one CU); the workgroup size is
 // chosen so that the workgroups that fit in LDS hold 16 wavefronts per CU.
 // STRIDE and SIN as for interior(): the program flags the variant requires.
-// TL > 0: the triaxial program of degree TL, unrolled (interior()); its
-// register use is held to the occupancy the LDS allows anyway (4 waves per
-// SIMD, 3 for the Hessian), or the unrolled steps' loads would all be
-// scheduled first.  The potential alone is held to 64 VGPRs (8 waves: it
-// runs in workgroups twice as large).
-template <int ORD, int BLOCK, int STRIDE, bool SIN, int TL>
+// TL > 0 / FL > 0: the triaxial / full program of that degree, unrolled
+// (interior()); its register use is held to the occupancy the LDS allows
+// anyway (4 waves per SIMD, 3 for the Hessian), or the unrolled steps' loads
+// would all be scheduled first.  The potential alone is held to 64 VGPRs (8
+// waves: it runs in workgroups twice as large).
+template <int ORD, int TL, int FL>
 constexpr int lds_min_waves()
 {
-    return ORD == ORD_POT ? 8 : TL == 0 ? 1 : ORD == ORD_HESS ? 3 : 4;
+    return ORD == ORD_POT ? 8 : TL == 0 && FL == 0 ? 1 : ORD == ORD_HESS ? 3 : 4;
 }
-template <int ORD, int BLOCK, int STRIDE, bool SIN, int TL = 0>
-__global__ __launch_bounds__(BLOCK, (lds_min_waves<ORD, BLOCK, STRIDE, SIN, TL>())) void mp_lds_kernel(
+template <int ORD, int BLOCK, int STRIDE, bool SIN, int TL = 0, int FL = 0>
+__global__ __launch_bounds__(BLOCK, (lds_min_waves<ORD, TL, FL>())) void mp_lds_kernel(
     MPK P, const double *__restrict__ xyz, int64_t n, double *__restrict__ phi,
     double *__restrict__ force, double *__restrict__ deriv, double inv4piG)
 {
@@ -74,8 +74,9 @@ __global__ __launch_bounds__(BLOCK, (lds_min_waves<ORD, BLOCK, STRIDE, SIN, TL>(
         PointOut o{};
         Where w = classify(P, x, y, z);
         if (w.kind == 2)
-            interior<ORD, false, NoHook, STRIDE, SIN, TL>(P, lds_tab + w.k * slab_len, x, y, z,
-                                                          w.r2, w.r, w.ir, w.lnr, w.k, o, inv4piG);
+            interior<ORD, false, NoHook, STRIDE, SIN, TL, FL>(P, lds_tab + w.k * slab_len, x, y,
+                                                              z, w.r2, w.r, w.ir, w.lnr, w.k, o,
+                                                              inv4piG);
         else
             extrap<ORD>(P, w, x, y, z, o, inv4piG);
         store<ORD>(o, i, phi, force, deriv);
@@ -83,33 +84,38 @@ __global__ __launch_bounds__(BLOCK, (lds_min_waves<ORD, BLOCK, STRIDE, SIN, TL>(
 }
 
 // Dynamic LDS beyond 64 KB has to be allowed per kernel.
-template <int ORD, int BLOCK, int STRIDE, bool SIN, int TL>
+template <int ORD, int BLOCK, int STRIDE, bool SIN, int TL, int FL>
 static void lds_attr(int64_t bytes)
 {
     static int64_t set = 0;
     if (bytes > set) {
-        if (hipFuncSetAttribute((const void *)mp_lds_kernel<ORD, BLOCK, STRIDE, SIN, TL>,
+        if (hipFuncSetAttribute((const void *)mp_lds_kernel<ORD, BLOCK, STRIDE, SIN, TL, FL>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsPerCu) ==
             hipSuccess)
             set = kLdsPerCu;
     }
 }
 
-template <int ORD, int BLOCK, int STRIDE, bool SIN, int TL = 0>
+template <int ORD, int BLOCK, int STRIDE, bool SIN, int TL = 0, int FL = 0>
 static void launch_lds(const MPK &P, unsigned grid, int64_t tb, const double *xyz, int64_t n,
                        double *phi, double *force, double *deriv, double inv4piG, hipStream_t s)
 {
-    if (BLOCK > 256) lds_attr<ORD, BLOCK, STRIDE, SIN, TL>(tb);
-    hipLaunchKernelGGL((mp_lds_kernel<ORD, BLOCK, STRIDE, SIN, TL>), dim3(grid), dim3(BLOCK),
+    if (BLOCK > 256) lds_attr<ORD, BLOCK, STRIDE, SIN, TL, FL>(tb);
+    hipLaunchKernelGGL((mp_lds_kernel<ORD, BLOCK, STRIDE, SIN, TL, FL>), dim3(grid), dim3(BLOCK),
                        (size_t)tb, s, P, xyz, n, phi, force, deriv, inv4piG);
 }
 
-// The unrolled kernel of the triaxial program of degree tl (even, 2..10) for
+// The unrolled kernel of the triaxial program of degree tl (even, 2..12) for
 // this evaluation order and workgroup size (tables up to 80 KB); false
 // (nothing launched) where no such variant is built: the caller then runs
 // the generic one.  multipole_tri.hip.
 bool launch_lds_triaxial(int ord, int block, int tl, const MPK &P, unsigned grid, int64_t tb,
                          const double *xyz, int64_t n, double *phi, double *force, double *deriv,
                          double inv4piG, hipStream_t s);
+// The same for the full program of degree fl (even, 2..10; every workgroup
+// size: tables up to 160 KB; multipole_full.hip).
+bool launch_lds_full(int ord, int block, int fl, const MPK &P, unsigned grid, int64_t tb,
+                     const double *xyz, int64_t n, double *phi, double *force, double *deriv,
+                     double inv4piG, hipStream_t s);
 
 }  // namespace agx

@@ -1,10 +1,8 @@
 // The unrolled LDS-staged Multipole kernels of the triaxial programs
-// (interior() with TL > 0): degrees 2..10, every evaluation order, in the
-// workgroup sizes of tables up to 80 KB (larger ones run the generic
-// kernels).  Degree 12 is left to the generic kernels: unrolled, its 28 steps
-// ran 3 % (potential) to 13 % (Hessian) slower than the loop at N = 10M.
-// A translation unit of its own: these instantiations compile in parallel
-// with multipole.hip.
+// (interior() with TL > 0): degrees 2..12 (the potential alone 2..10), every
+// evaluation order, in the workgroup sizes of tables up to 80 KB (larger ones
+// run the generic kernels).  A translation unit of its own: these
+// instantiations compile in parallel with multipole.hip.
 #include "multipole_lds.h"
 
 namespace agx {
@@ -14,6 +12,8 @@ static bool tri_launch(int tl, const MPK &P, unsigned grid, int64_t tb, const do
                        int64_t n, double *phi, double *force, double *deriv, double inv4piG,
                        hipStream_t s)
 {
+    // (the potential alone at degree 12 ran 4 % slower unrolled)
+    if (ORD == ORD_POT && tl == 12) return false;
     switch (tl) {
 #define AGX_TRI_CASE(L)                                                                        \
     case L:                                                                                    \
@@ -24,6 +24,7 @@ static bool tri_launch(int tl, const MPK &P, unsigned grid, int64_t tb, const do
         AGX_TRI_CASE(6)
         AGX_TRI_CASE(8)
         AGX_TRI_CASE(10)
+        AGX_TRI_CASE(12)
 #undef AGX_TRI_CASE
     default:
         return false;

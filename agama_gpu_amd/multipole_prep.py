@@ -270,18 +270,43 @@ def triaxial_degree(prog: LegendreProgram) -> int:
     return L if ok else 0
 
 
+def full_degree(prog: LegendreProgram) -> int:
+    """L when the program is the full program of degree L >= 1: groups
+    m = 0, 1, .., L, group m stepping l = m, m + 1, .., L by 1, the cos term
+    of step j in column j for 1 <= j <= L (step 0, l = m = 0, has none: the
+    monopole is evaluated apart) and for a later step j the cos term in
+    column 2j - L - 1 and the sin term in column 2j - L -- the program of a
+    table with every harmonic up to degree L.  0 otherwise."""
+    G = len(prog.grp_m)
+    L = G - 1
+    if G < 2 or not np.array_equal(prog.grp_m, np.arange(L + 1)):
+        return 0
+    step_l = np.concatenate([np.arange(m, L + 1) for m in range(L + 1)])
+    S = len(step_l)
+    step0 = np.concatenate([[0], np.cumsum([L - m + 1 for m in range(L + 1)])])
+    j = np.arange(S)
+    cos = np.where(j <= L, j, 2 * j - L - 1)
+    cos[0] = -1
+    sin = np.where(j <= L, -1, 2 * j - L)
+    ok = (np.array_equal(prog.grp_step0, step0) and np.array_equal(prog.step_l, step_l)
+          and np.array_equal(prog.step_cos, cos) and np.array_equal(prog.step_sin, sin)
+          and np.all(prog.grp_stride[np.diff(prog.grp_step0) > 1] == 1))
+    return L if ok else 0
+
+
 def program_flags(prog: LegendreProgram) -> int:
     """agx_multipole.prog_flags for this program: AGX_PROG_STRIDE1 (1) /
     AGX_PROG_STRIDE2 (4) when every group that recurs (more than one step)
     has stride 1 / 2, plus AGX_PROG_COS_ONLY (2) when no step has a sin term,
-    plus the triaxial degree (triaxial_degree) in bits 8-15."""
+    plus the triaxial degree (triaxial_degree) in bits 8-15 and the full
+    degree (full_degree) in bits 16-23."""
     steps = np.diff(prog.grp_step0)
     recurs = steps > 1
     stride1 = not np.any((prog.grp_stride != 1) & recurs)
     stride2 = not np.any((prog.grp_stride != 2) & recurs)
     cos_only = not np.any(prog.step_sin >= 0)
     return (int(stride1) | (2 if cos_only else 0) | (4 if stride2 else 0)
-            | (triaxial_degree(prog) << 8))
+            | (triaxial_degree(prog) << 8) | (full_degree(prog) << 16))
 
 
 # 32 bytes: four int32 (group records: m, step_begin, step_end, stride; step

@@ -91,7 +91,19 @@ enum {
  *             n_groups = L/2 + 1 and n_lm = n_steps = (L/2 + 1)(L/2 + 2)/2
  *             (checked: AGX_E_WHAT otherwise); the LDS-staged path then
  *             runs a kernel with that program unrolled at compile time
- *             (degrees 2..10, tables up to 80 KB; AGX_OPT_MP_TRIAXIAL).
+ *             (degrees 2..12, the potential alone 2..10, tables up to
+ *             80 KB; AGX_OPT_MP_TRIAXIAL);
+ *             or L << AGX_PROG_FULL_SHIFT (bits 16-23, L >= 1) when
+ *             `program` is the full program of degree L: groups m = 0, 1,
+ *             .. L, group m with the steps l = m, m + 1, .. L (stride 1),
+ *             cos_off = 6 j for 1 <= j <= L, -1 for step 0, and for a
+ *             later step j cos_off = 6 (2j - L - 1), sin_off = 6 (2j - L)
+ *             -- the program of a table with every harmonic up to l = L,
+ *             columns in program order (multipole_prep.full_degree).
+ *             Requires n_groups = L + 1, n_steps = (L + 1)(L + 2)/2 and
+ *             n_lm = (L + 1)^2, and no triaxial degree (checked); the
+ *             LDS-staged path then runs the unrolled program (even degrees
+ *             2..10, tables up to 160 KB).
  *             0 is always correct; a flag set for a program that does not
  *             satisfy it gives wrong results.  No counterpart in the
  *             reference (its kernel runs every table through the same loop).
@@ -101,6 +113,8 @@ enum {
 #define AGX_PROG_STRIDE2 4u
 #define AGX_PROG_TRIAXIAL_SHIFT 8
 #define AGX_PROG_TRIAXIAL_MASK 0xFF00u
+#define AGX_PROG_FULL_SHIFT 16
+#define AGX_PROG_FULL_MASK 0xFF0000u
 
 typedef struct agx_multipole {
     const double *poly_km;
@@ -461,9 +475,9 @@ int agx_leapfrog(const agx_component *d_leaves, int32_t n_leaves,
                                  blocks and the slabs fit in 53 KiB per
                                  workgroup, 0 six cross-lane sums per term */
 #define AGX_OPT_MP_TRIAXIAL 12 /* LDS-staged Multipole path: 1 (default) the
-                                 unrolled kernels of triaxial programs (the
-                                 degree in prog_flags bits 8-15: 2..10, tables
-                                 up to 80 KB) where the flags name one, 0 the
+                                 unrolled kernels of triaxial and full
+                                 programs where prog_flags names a degree
+                                 (bits 8-15 / 16-23) that has one, 0 the
                                  generic kernels for every program */
 int agx_set_option(int key, int64_t value);
 
@@ -481,7 +495,9 @@ int agx_set_option(int key, int64_t value);
  * program's size) and AGX_OPT_MP_TRIAXIAL; the LDS-staged table's slabs are
  * padded to an odd number of 16-byte words (fewer LDS bank conflicts for
  * tables with an even number of terms; the LDS path admits tables up to
- * 160 KB including the padding). */
+ * 160 KB including the padding); prog_flags bits 16-23 (the full degree,
+ * validated likewise); agx_batch_item.path bits 8-15 (the triaxial degree of
+ * an LDS item's potential). */
 int agx_version(void);
 const char *agx_error_string(int code);
 

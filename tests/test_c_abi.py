@@ -76,6 +76,13 @@ def test_argument_validation_without_gpu():
             (-1 if ok else -5), tl
     d.n_lm = 14
     assert L.agx_multipole_eval(C.byref(d), None, 10, None, None, None, 3, None) == -5
+    # a full degree (bits 16-23): degree 4 is 5 groups, 15 steps, 25 columns;
+    # not together with a triaxial degree
+    d.n_groups, d.n_steps, d.n_lm = 5, 15, 25
+    for flags, ok in ((4 << 16, True), (5 << 16, False), ((4 << 16) | (8 << 8), False)):
+        d.prog_flags = flags | _lib.AGX_PROG_STRIDE1
+        assert L.agx_multipole_eval(C.byref(d), None, 10, None, None, None, 3, None) == \
+            (-1 if ok else -5), hex(flags)
     d.prog_flags = 0
 
 

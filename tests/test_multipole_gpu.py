@@ -130,10 +130,12 @@ def test_program_flag_variants_agree(cases, name):
     from agama_gpu_amd.multipole_prep import program_flags
     gpu = MultipolePotentialGPU(cases[name])
     prog = gpu._data["program"]
-    # (halo_lmax8 is also the triaxial program of degree 8: bits 8-15, whose
-    # unrolled kernel tests/test_triaxial_gpu.py checks; off here)
+    # (halo_lmax8 is also the triaxial program of degree 8, bits 8-15, and the
+    # dense tables the full program of degree 8, bits 16-23, whose unrolled
+    # kernels tests/test_triaxial_gpu.py and tests/test_full_program_gpu.py
+    # check; off here)
     want = (_lib.AGX_PROG_COS_ONLY | _lib.AGX_PROG_STRIDE2 | (8 << 8)) \
-        if name == "halo_lmax8" else _lib.AGX_PROG_STRIDE1
+        if name == "halo_lmax8" else _lib.AGX_PROG_STRIDE1 | (8 << 16)
     assert program_flags(prog) == want == gpu._desc.prog_flags
     x = torch.as_tensor(points(100_000, 4), device="cuda")
     _lib.set_option(_lib.AGX_OPT_MP_PATH, 3)

@@ -136,3 +136,38 @@ def test_triaxial_degree_rejects_other_programs():
     assert triaxial_degree(prog_of(lambda a, b: tri(a, b) or (a, b) == (4, -2))) == 0
     assert triaxial_degree(prog_of(lambda a, b: tri(a, b) or (a, b) == (3, 0))) == 0
     assert triaxial_degree(prog_of(lambda a, b: a == 0)) == 0
+
+
+@pytest.mark.parametrize("lmax", [1, 2, 5, 8, 10, 32])
+def test_full_degree_of_full_programs(lmax):
+    """Every harmonic up to lmax: the full program of that degree, whose
+    group g starts at step full_step0(L, g) = g (L + 1) - g (g - 1) / 2, with
+    the cos term of step j in column j (j <= L) or 2j - L - 1 and the sin term
+    in column 2j - L; in program_flags bits 16-23, and no triaxial degree."""
+    from agama_gpu_amd.multipole_prep import full_degree, program_flags, triaxial_degree
+    l, m, prog = _program(lmax, "none")
+    assert full_degree(prog) == lmax and triaxial_degree(prog) == 0
+    assert program_flags(prog) >> 16 == lmax and (program_flags(prog) >> 8) & 0xFF == 0
+    G = lmax + 1
+    assert list(prog.grp_step0) == [g * (lmax + 1) - g * (g - 1) // 2 for g in range(G + 1)]
+
+
+def test_full_degree_rejects_other_programs():
+    """A triaxial program, a full program with one term missing or a sin term
+    missing, the monopole alone: no full degree."""
+    from agama_gpu_amd.expansion import lm_labels
+    from agama_gpu_amd.multipole_prep import _sort_key_order, build_legendre_program, full_degree
+
+    def prog_of(keep, lmax=6):
+        lab = [(a, b) for a, b in lm_labels(lmax) if keep(a, b)]
+        l = np.array([a for a, _ in lab], np.int32)
+        m = np.array([b for _, b in lab], np.int32)
+        o = _sort_key_order(l, m)
+        return build_legendre_program(l[o], m[o])
+
+    assert full_degree(prog_of(lambda a, b: True)) == 6
+    assert full_degree(prog_of(lambda a, b: a % 2 == 0 and b % 2 == 0 and b >= 0)) == 0
+    assert full_degree(prog_of(lambda a, b: (a, b) != (4, 2))) == 0
+    assert full_degree(prog_of(lambda a, b: (a, b) != (3, -1))) == 0
+    assert full_degree(prog_of(lambda a, b: (a, b) != (6, 6))) == 0
+    assert full_degree(prog_of(lambda a, b: a == 0)) == 0

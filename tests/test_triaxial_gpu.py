@@ -3,8 +3,9 @@
 A table holding every harmonic that triaxial symmetry allows up to an even
 degree L (l even, m even and >= 0) has a fixed program (triaxial_degree);
 the host names L in prog_flags bits 8-15 and the LDS path runs a kernel with
-that program unrolled at compile time (degrees 2..10; 12 and tables above
-80 KB run the generic kernels).  Checked here, on the LDS path (N >= 4096):
+that program unrolled at compile time (degrees 2..12, the potential alone
+2..10; tables above 80 KB run the generic kernels).  Checked here, on the
+LDS path (N >= 4096):
 
 * for triaxial Dehnen tables of degree 2..12, every evaluation order against
   the generic kernel of the same table (AGX_OPT_MP_TRIAXIAL 0) and against
