@@ -168,6 +168,7 @@ __host__ __device__ constexpr int full_step0(int L, int g)
 // the full program of degree FL (full_step0): its groups, steps and column
 // offsets are compile-time constants, so the group and step loops unroll
 // completely; the records supply only the recurrence coefficients and q_mm.
+// The slab must then be in LDS (the loads are LDS loads at immediate offsets).
 template <int ORD, bool UNI, class Hook = NoHook, int STRIDE = 0, bool SIN = true, int TL = 0,
           int FL = 0>
 __device__ __forceinline__ void interior(const MPK &P, const double *slab, double x, double y,
@@ -244,10 +245,11 @@ __device__ __forceinline__ void interior(const MPK &P, const double *slab, doubl
     constexpr int kUnrollPairs = TL > 0 ? TL / 4 + 1 : FL > 0 ? FL / 2 + 1 : 1;
     const int n_groups = CT ? kUnrollGroups : P.n_groups;
     const int S0 = n_groups + 1;
-    // (CT: the records' addresses are constants; re-derived per point, their
-    // loads stay here instead of being hoisted out of the caller's loop)
-    // (each unrolled step re-derives it again, below: the next steps' records
-    // are then read one step ahead, not all at the start)
+    // Unrolled (CT), the records' addresses are constants: the pointer is
+    // re-derived per point, so that their loads stay here instead of being
+    // hoisted out of the caller's particle loop, and again after every step
+    // (in terms()), so that each record is read one step ahead and not all
+    // of them at the start.
     const void *prog = CT ? sopaque(P.prog) : P.prog;
     static_assert(!(TL > 0 && FL > 0) && (!CT || !UNI), "one compile-time program, LDS slabs");
     static_assert(TL == 0 || (STRIDE == 2 && !SIN), "the triaxial program: stride 2, cos terms");
