@@ -1,9 +1,9 @@
-// The unrolled LDS-staged Multipole kernels of full programs (interior()
-// with FL > 0: every harmonic up to l = FL, cos and sin, as in a table
-// without symmetry), in every evaluation order and in the workgroup sizes the
-// LDS path uses.  A translation unit of its own: these instantiations
+// The unrolled Multipole kernels of full programs (interior() with FL > 0:
+// every harmonic up to l = FL, cos and sin, as in a table without symmetry),
+// in every evaluation order: the LDS-staged kernel in the workgroup sizes the
+// LDS path uses, and the interval-sorted kernel with LDS slabs.  A translation unit of its own: these instantiations
 // compile in parallel with the rest.
-#include "multipole_lds.h"
+#include "multipole_sorted.h"
 
 namespace agx {
 
@@ -44,6 +44,37 @@ static bool full_launch(int block, int fl, const MPK &P, unsigned grid, int64_t 
     if (block == B3)
         return full_launch<ORD, B3>(fl, P, grid, tb, xyz, n, phi, force, deriv, inv4piG, s);
     return false;
+}
+
+template <int ORD>
+static bool sorted_full(int fl, const MPK &P, const SortedLaunch &L, size_t lds, hipStream_t s)
+{
+    switch (fl) {
+#define AGX_SORTED_FULL_CASE(F)                                  \
+    case F:                                                      \
+        launch_sorted<ORD, true, 1, true, 0, F>(P, L, lds, s);   \
+        return true;
+        AGX_SORTED_FULL_CASE(2)
+        AGX_SORTED_FULL_CASE(4)
+        AGX_SORTED_FULL_CASE(6)
+        AGX_SORTED_FULL_CASE(8)
+        AGX_SORTED_FULL_CASE(10)
+#undef AGX_SORTED_FULL_CASE
+    default:
+        return false;
+    }
+}
+
+bool launch_sorted_full(int ord, int fl, const MPK &P, const SortedLaunch &L, size_t lds,
+                        hipStream_t s)
+{
+    switch (ord) {
+    case ORD_POT: return sorted_full<ORD_POT>(fl, P, L, lds, s);
+    case ORD_GRAD: return sorted_full<ORD_GRAD>(fl, P, L, lds, s);
+    case ORD_HESS: return sorted_full<ORD_HESS>(fl, P, L, lds, s);
+    case ORD_DENS: return sorted_full<ORD_DENS>(fl, P, L, lds, s);
+    default: return false;
+    }
 }
 
 bool launch_lds_full(int ord, int block, int fl, const MPK &P, unsigned grid, int64_t tb,
