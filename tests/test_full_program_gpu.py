@@ -5,7 +5,8 @@ nothing pruned: the reference's own FIRE and LMC fixtures, lmax 8 on 25
 nodes, and Agama's files of that kind generally) has a fixed program
 (full_degree); the host names L in prog_flags bits 16-23 and the LDS path
 runs a kernel with that program unrolled at compile time (even degrees
-2..10).  Checked here, on the LDS path (N >= 4096), every evaluation order:
+2..10).  The interval-sorted kernel's LDS-slab variant runs the same unrolled programs.
+Checked here, on the LDS path and on the sorted path, every evaluation order:
 
 * the two fixtures and random self-consistent full tables of degree 2..10
   (tests/random_tables.py, 19..26 nodes so that they fit in LDS) against the
@@ -27,6 +28,15 @@ from test_triaxial_gpu import check, evaluate, lds_path, sample  # noqa: F401
 
 pytestmark = pytest.mark.gpu
 
+
+@pytest.fixture(params=[3, 4], ids=["lds", "sorted"])
+def lds_or_sorted(request):
+    from agama_gpu_amd import _lib
+    _lib.set_option(_lib.AGX_OPT_MP_PATH, request.param)
+    yield request.param
+    _lib.set_option(_lib.AGX_OPT_MP_PATH, 0)
+    _lib.set_option(_lib.AGX_OPT_MP_TRIAXIAL, 1)
+
 FIXTURES = {"fire": "600.dark.none_8.coef_mul_DR", "lmc": "100.LMC.none_8.coef_mult"}
 
 
@@ -41,7 +51,7 @@ def full_table(L, seed0):
 
 
 @pytest.mark.parametrize("name", list(FIXTURES))
-def test_full_program_fixtures(lds_path, name):
+def test_full_program_fixtures(lds_or_sorted, name):
     from pathlib import Path
 
     from agama_gpu_amd import MultipolePotentialGPU, _lib
@@ -49,18 +59,19 @@ def test_full_program_fixtures(lds_path, name):
     c = read_coefs(Path(__file__).parent / "golden" / FIXTURES[name])
     gpu = MultipolePotentialGPU(c)
     degree = (gpu._desc.prog_flags & _lib.AGX_PROG_FULL_MASK) >> _lib.AGX_PROG_FULL_SHIFT
-    print(f"case full fixture {name} flags={gpu._desc.prog_flags:#x}", flush=True)
+    print(f"case full fixture {name} flags={gpu._desc.prog_flags:#x} path={lds_or_sorted}",
+          flush=True)
     assert degree == 8 and gpu._desc.n_lm == 81
     check(gpu, c, sample(20_000, float(c.R_grid[-1]), 3), name)
 
 
 @pytest.mark.parametrize("L", [2, 4, 6, 8, 10])
-def test_full_program_random_tables(lds_path, L):
+def test_full_program_random_tables(lds_or_sorted, L):
     from agama_gpu_amd import MultipolePotentialGPU, _lib
     c, d = full_table(L, 100 * L)
     gpu = MultipolePotentialGPU(c)
     degree = (gpu._desc.prog_flags & _lib.AGX_PROG_FULL_MASK) >> _lib.AGX_PROG_FULL_SHIFT
-    print(f"case full L={L} {d}", flush=True)
+    print(f"case full L={L} path={lds_or_sorted} {d}", flush=True)
     assert degree == L and gpu._desc.n_lm == (L + 1) ** 2
     check(gpu, c, sample(20_000, d["r_max"], L), d)
 
