@@ -115,6 +115,30 @@ def test_triaxial_512_thread_variant(tables, lds_path):
     check(gpu, c, sample(30_000, float(c.R_grid[-1]), 5), "nR=100")
 
 
+def test_triaxial_bad_positions(tables, lds_path):
+    """NaN in any coordinate gives NaN outputs, r^2 = inf gives 0 (the outer
+    law at r = inf), as on the other kernels; the good points around them
+    are those of the same launch without the bad ones, bit for bit."""
+    from agama_gpu_amd import MultipolePotentialGPU, _lib
+    gpu = MultipolePotentialGPU(tables[8])
+    _lib.set_option(_lib.AGX_OPT_MP_TRIAXIAL, 1)
+    good = sample(8192, float(tables[8].R_grid[-1]), 11)
+    bad = np.array([[np.nan, 1.0, 2.0], [1.0, np.nan, 2.0], [1.0, 2.0, np.nan],
+                    [1e200, 0.0, 0.0], [0.0, -1e200, 3.0], [1.0, 1.0, 1e170]])
+    isnan = np.array([True, True, True, False, False, False])
+    print("case triaxial L=8 bad positions", flush=True)
+    x = torch.as_tensor(np.concatenate([good, bad, good[:100]]), device="cuda")
+    p, F, D = [t.cpu().numpy() for t in gpu.evalDeriv(x)]
+    rp, rF, rD = [t.cpu().numpy() for t in gpu.evalDeriv(torch.as_tensor(good, device="cuda"))]
+    b = slice(8192, 8192 + len(bad))
+    assert np.all(np.isnan(p[b][isnan])) and np.all(np.isnan(F[b][isnan]).any(1))
+    assert np.all(np.isnan(D[b][isnan]).any(1))
+    assert np.all(p[b][~isnan] == 0.0) and np.all(F[b][~isnan] == 0.0)
+    assert np.all(D[b][~isnan] == 0.0)
+    assert np.array_equal(p[:8192], rp) and np.array_equal(F[:8192], rF)
+    assert np.array_equal(D[:8192], rD)
+
+
 def test_pruned_triaxial_table_runs_generic(tables, lds_path):
     """One harmonic zeroed (pruned from the program): no triaxial degree, and
     the option changes nothing."""
