@@ -56,6 +56,37 @@ def test_batched_equals_per_potential(ensemble_parts, path, rows, order):
         assert _rel(a, b) < 1e-13
 
 
+@pytest.mark.parametrize("path,rows", [(0, 4096), (1, 8192)])
+def test_batched_bad_positions(ensemble_parts, path, rows):
+    """NaN coordinates give NaN outputs and r^2 = inf gives 0 in every
+    potential's rows, as in one launch per potential; the rows around them
+    agree with that launch as in the test above."""
+    from agama_gpu_amd.batch import MultipoleEnsembleGPU
+    _, pots, counts, x = ensemble_parts
+    x = x.copy()
+    rng = np.random.default_rng(17)
+    bad = rng.choice(len(x), 64, replace=False)
+    isnan = np.arange(64) % 2 == 0
+    x[bad[isnan], rng.integers(0, 3, isnan.sum())] = np.nan
+    x[bad[~isnan]] = [1e200, -1e200, 1e170] * np.sign(rng.standard_normal((32, 3)))
+    print(f"case bad positions path={path} rows={rows}", flush=True)
+    ens = MultipoleEnsembleGPU(pots, counts, choices=[(path, rows)] * len(pots))
+    xyz = torch.as_tensor(x, device=DEV)
+    b = [t.cpu().numpy() for t in ens.eval(xyz, der=True)]
+    n = len(x)
+    mk = lambda *s: torch.full(s, 7.0, dtype=torch.float64, device=DEV)  # noqa: E731
+    s = (mk(n), mk(n, 3), mk(n, 6))
+    ens.eval_separately(xyz, *s)
+    s = [t.cpu().numpy() for t in s]
+    good = np.ones(n, bool)
+    good[bad] = False
+    for a, r in zip(b, s):
+        a2, r2 = a.reshape(n, -1), r.reshape(n, -1)
+        assert np.all(np.isnan(a2[bad[isnan]]).any(1)) and np.all(np.isnan(r2[bad[isnan]]).any(1))
+        assert np.all(a2[bad[~isnan]] == 0.0) and np.all(r2[bad[~isnan]] == 0.0)
+        assert np.max(np.abs(a2[good] - r2[good])) <= 1e-13 * np.max(np.abs(r2[good]))
+
+
 def test_batched_against_oracle(ensemble_parts):
     from agama_gpu_amd.batch import MultipoleEnsembleGPU
     from oracle.oracle import OracleMultipole
