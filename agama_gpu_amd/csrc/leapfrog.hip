@@ -5,9 +5,12 @@
 // factors) is tabulated per step by the host into a schedule: for step s the
 // potential at time t0 + (s+1) dt is the list of leaves d_sched[s][0..count[s])
 // -- a leaf index into d_leaves and the weight, scale and centre of that leaf
-// at that time.  The step counter lives in device memory: each step kernel
-// reads it and a one-thread kernel advances it, so a graph captured once
-// replays step after step with the time moving on and no host work.
+// at that time.  The step counter lives in device memory: the j-th step
+// kernel of a call runs step counter + j, and one one-thread kernel after the
+// call's last step advances the counter by the call's step count, so a graph
+// captured once replays step after step with the time moving on and no host
+// work.  (A counter kernel after every step would put a dependency gap between
+// launches, about 4.7 us, into every step.)
 //
 // One launch per step does, per particle, v += dt/2 a; x += dt v;
 // a = F(x, t_{s+1}); v += dt/2 a -- one read and one write of (x, v, a).
@@ -20,11 +23,12 @@ namespace agx {
 __global__ __launch_bounds__(kBlock) void leapfrog_kernel(
     const agx_component *__restrict__ leaves, int n_leaves, const agx_sched_entry *__restrict__ sched,
     const int *__restrict__ count, int max_active, int64_t n_sched, const int *__restrict__ d_step,
-    double *__restrict__ xs, double *__restrict__ vs, double *__restrict__ as, int64_t n, double dt)
+    int j, double *__restrict__ xs, double *__restrict__ vs, double *__restrict__ as, int64_t n,
+    double dt)
 {
-    const int s = __builtin_amdgcn_readfirstlane(cldi(d_step, 0));
+    const int64_t s = (int64_t)__builtin_amdgcn_readfirstlane(cldi(d_step, 0)) + j;
     if (s < 0 || s >= n_sched) return;          // past the schedule: particles stay put
-    int cnt = cldi(count, s);
+    int cnt = cldi(count, (int)s);          // s < n_sched: a schedule row
     if (cnt > max_active) cnt = max_active;
     const double hdt = 0.5 * dt;
     const int64_t stride = (int64_t)gridDim.x * kBlock;
@@ -52,9 +56,9 @@ __global__ __launch_bounds__(kBlock) void leapfrog_kernel(
     }
 }
 
-__global__ void leapfrog_advance(int *d_step)
+__global__ void leapfrog_advance(int *d_step, int by)
 {
-    if (threadIdx.x == 0) d_step[0] = d_step[0] + 1;
+    if (threadIdx.x == 0) d_step[0] = d_step[0] + by;
 }
 
 }  // namespace agx
@@ -75,8 +79,8 @@ extern "C" int agx_leapfrog(const agx_component *d_leaves, int32_t n_leaves,
     for (int32_t k = 0; k < n_steps; ++k) {
         hipLaunchKernelGGL(leapfrog_kernel, dim3(grid), dim3(kBlock), 0, s, d_leaves,
                            (int)n_leaves, d_sched, (const int *)d_count, (int)max_active, n_sched,
-                           (const int *)d_step, d_x, d_v, d_a, n, dt);
-        hipLaunchKernelGGL(leapfrog_advance, dim3(1), dim3(64), 0, s, (int *)d_step);
+                           (const int *)d_step, (int)k, d_x, d_v, d_a, n, dt);
     }
+    hipLaunchKernelGGL(leapfrog_advance, dim3(1), dim3(64), 0, s, (int *)d_step, (int)n_steps);
     return hipGetLastError() == hipSuccess ? AGX_OK : AGX_E_LAUNCH;
 }

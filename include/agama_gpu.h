@@ -412,10 +412,11 @@ int agx_multipole_batch_eval(const void *d_desc, int64_t desc_stride, int32_t n_
  * leaf of d_leaves (type, parameters and table of an agx_component; its own
  * ampl/scale/center fields are ignored) and giving that leaf's weight, scale
  * and centre at that time (the flattening described for agx_component).
- * d_step is a device int32 step counter: every step reads it and then
- * advances it by one (by a kernel), so replaying a captured graph of
- * agx_leapfrog launches continues the integration.  Steps with a counter
- * outside [0, n_sched) leave the particles untouched.
+ * d_step is a device int32 step counter: the j-th of a call's n_steps steps
+ * runs step *d_step + j, and after the last one a kernel advances *d_step by
+ * n_steps, so replaying a captured graph of agx_leapfrog launches continues
+ * the integration.  Steps outside [0, n_sched) leave the particles
+ * untouched.
  * d_x, d_v, d_a [N][3]: positions, velocities and the acceleration at the
  * current positions and time (the caller evaluates it before the first
  * step); all three are updated in place.  No reference counterpart (the

@@ -56,6 +56,23 @@ def test_resume(problem):
     assert _rel(lf.x, xe) < 1e-11 and _rel(lf.v, ve) < 1e-11
 
 
+def test_steps_past_schedule_leave_particles(problem):
+    """A call that runs past the schedule (step counter + j >= n_sched) moves
+    the particles only for its scheduled steps; the counter still advances by
+    the call's step count."""
+    from agama_gpu_amd import LeapfrogGPU
+    pot, x, v = problem
+    lf = LeapfrogGPU(pot, dt=2e-3, n_steps=12, t0=0.0)
+    lf.run(x, v, n_steps=9, graph=False)
+    lf.advance(6, graph=False)                     # steps 9..11 run, 12..14 do nothing
+    assert lf.steps_done == 15
+    xe, ve = lf.run_eager(x, v, 12)
+    assert _rel(lf.x, xe) < 1e-11 and _rel(lf.v, ve) < 1e-11
+    x_end = lf.x.clone()
+    lf.advance(4, graph=True, steps_per_graph=4)
+    assert lf.steps_done == 19 and torch.equal(lf.x, x_end)
+
+
 def _energy(pot, x, v):
     return pot.potential(x) + 0.5 * (v * v).sum(1)
 
